@@ -1,0 +1,149 @@
+/*
+ * mtcocr.h — C-ABI of the MI355X COCR solver (drop-in replacement of the reference's COCR_FP).
+ *
+ * The reference solves one complex-symmetric system A x = b per (frequency, polarization) with
+ * COCR, the folded ("DFP") preconditioner y = D_A^-1 x + G^T D_N^-1 G x and a minimal-residual
+ * smoothing stop test. Entry points and the reference interfaces they replace:
+ *
+ *   mtcocr_pcocr_2x2_folded   PCOCR::PCOCR_2x2_Folded          Code/COCR_FP/pcocr.h:27-31,
+ *                                                                  pcocr.cpp:40-126
+ *   mtcocr_set_matrix_block2x2  the SLAE arrays that main() passes to it
+ *                                                                  Code/COCR_FP/HarmVect3d.cpp:196-229
+ *   mtcocr_set_gradient_mesh  FoldedPreconditioner::BuildGMatrix   FoldedPreconditioner.cpp:68-259
+ *   mtcocr_solve              the request loop of PCOCR_2x2_Folded pcocr.cpp:61-123
+ *   mtcocr_set_matrix_ks / mtcocr_solve_batch / mtcocr_set_rhs / mtcocr_solve_resident /
+ *   mtcocr_get_solution       no single counterpart: they batch the reference's per-(f, pol)
+ *                             COCR_FP processes (Code/CalcStarter/main.cpp:291-325) over one
+ *                             shared sparsity, A_s = K + i*omega_s*S.
+ *   mtcocr_bench_spmv         no counterpart (measurement hook for the SpMV kernel).
+ *
+ * Conventions: all pointers are host pointers to caller-owned memory; indices are 0-based int32
+ * (the reference decrements its 1-based files before the call, HarmVect3d.cpp:203-226); complex
+ * vectors are interleaved (re, im) float64, length 2*nb; x is in/out (initial guess on entry).
+ * Matrix layout = the reference's lower-triangle 2x2-block storage (Code/COCR_FP/
+ * block_2x2_solver.cpp:73-105): ig[nb+1], jg[nnzL], idi[nb+1] into di, ijg[nnzL+1] into gg,
+ * 1 real per block (real coupling) or 2 reals (re, im). A = L + L^T + D, no conjugation.
+ *
+ * Return codes: the reference's request codes (Code/COCR_FP/rci.h:114-133) for solver outcomes,
+ * negative MTCOCR_ERR_* for argument, device and allocation errors. Nothing throws across the
+ * ABI. A handle owns one HIP stream and its device memory; do not share a handle between
+ * threads.
+ */
+#ifndef MTCOCR_H
+#define MTCOCR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTCOCR_OK 0            /* REQ_OK: converged */
+#define MTCOCR_X0_NULL (-1)    /* REQ_X0_NULL: ||b - A x0|| < 1e-30 */
+#define MTCOCR_DIV_BY_ZERO (-2)/* REQ_DIV_BY_ZERO: (a, w) == 0 */
+#define MTCOCR_MAXITER (-3)    /* REQ_MAXITER */
+#define MTCOCR_ERR_ARG (-10)
+#define MTCOCR_ERR_HIP (-11)
+#define MTCOCR_ERR_ALLOC (-12)
+#define MTCOCR_ERR_STATE (-13) /* call order: matrix / gradient / rhs missing */
+#define MTCOCR_ERR_NOTCONV (-14) /* mixed precision: refinement stalled above the tolerance */
+
+#define MTCOCR_PREC_FP64 0  /* plain FP64 COCR, the reference's iteration and stop test */
+#define MTCOCR_PREC_MIXED 1 /* complex FP32 inner COCR + FP64 iterative refinement */
+
+typedef struct mtcocr_handle mtcocr_handle;
+
+typedef struct mtcocr_stats {
+  int req;                /* request code of this system */
+  int iter;               /* COCR iterations (mixed: sum of inner iterations) */
+  int outer;              /* refinement passes (FP64 path: 1) */
+  int reserved;
+  double residual_rel;    /* last MRS-smoothed relative residual (reference's residualRel) */
+  double true_rel_res;    /* ||b - A x|| / ||b|| in FP64, recomputed on the device */
+  double seconds;         /* device time of DFP setup + iterations (the reference's kit span) */
+  double seconds_io;      /* host<->device copies of b and x */
+} mtcocr_stats;
+
+const char* mtcocr_error_string(int code);
+
+/* Handle lifetime. device = HIP device ordinal. */
+int mtcocr_create(int device, mtcocr_handle** out);
+int mtcocr_destroy(mtcocr_handle* h);
+
+/* One system in the reference's block layout (uploads a device copy in the solver's own layout,
+ * FP64 and FP32). */
+int mtcocr_set_matrix_block2x2(mtcocr_handle* h, int nb, const int* ig, const int* jg,
+                               const int* idi, const int* ijg, const double* di,
+                               const double* gg);
+
+/* Frequency-independent pair on the same lower-triangle portrait (one real per block):
+ * A(omega) = K + i*omega*S. S must vanish on blocks of real (orthogonal-edge) couplings. */
+int mtcocr_set_matrix_ks(mtcocr_handle* h, int nb, const int* ig, const int* jg,
+                         const double* k_di, const double* k_gg, const double* s_di,
+                         const double* s_gg);
+
+/* Discrete gradient G of the folded preconditioner from the mesh, exactly as BuildGMatrix:
+ * nver[kpar*14] (0-based nodes), nvkat[kpar] (0-based materials), sigma3d[materials]
+ * (a node is kept when it belongs to a cell with sigma3d > 0 and is not a boundary node),
+ * edges[n_edges*2] (0-based end nodes), optional hanging-node map ig_t/jg_t/gg_t (NULL when
+ * n_nodes_c covers every node), is_node_bound[n_nodes]. */
+int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_nodes_c,
+                             const int* nvkat, const int* nver, const double* sigma3d,
+                             const int* edges, const int* ig_t, const int* jg_t,
+                             const double* gg_t, const int* is_node_bound);
+
+/* Solve one system set by mtcocr_set_matrix_block2x2. pr = b (2*nb), x in/out. */
+int mtcocr_solve(mtcocr_handle* h, const double* pr, double* x, double eps, int maxiter,
+                 int precision, mtcocr_stats* stats);
+
+/* Batched solve of nsys systems A(omega_s) x_s = b_s sharing the matrix set by
+ * mtcocr_set_matrix_ks (or, with omega == NULL and nsys == 1, the block2x2 matrix).
+ * pr and x hold nsys consecutive vectors of 2*nb doubles; stats has nsys entries.
+ * Systems retire independently. Returns 0 when every system returned MTCOCR_OK, otherwise the
+ * first non-zero per-system code. */
+int mtcocr_solve_batch(mtcocr_handle* h, int nsys, const double* omega, const double* pr,
+                       double* x, double eps, int maxiter, int precision, mtcocr_stats* stats);
+
+/* Resident-input form of mtcocr_solve_batch, for callers that time the device work alone:
+ * set_rhs copies b (and x0) to the device, solve_resident runs the solver on device-resident
+ * data and waits for it, get_solution copies x back. */
+int mtcocr_set_rhs(mtcocr_handle* h, int nsys, const double* omega, const double* pr,
+                   const double* x0);
+int mtcocr_solve_resident(mtcocr_handle* h, double eps, int maxiter, int precision,
+                          mtcocr_stats* stats);
+int mtcocr_get_solution(mtcocr_handle* h, double* x);
+
+/* One-shot form with the argument list of PCOCR::PCOCR_2x2_Folded minus y_omp (the reference's
+ * per-thread scatter buffer) and with nver/edges as flat arrays. Runs on device 0 (or the device
+ * named by MTCOCR_DEVICE). */
+int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi, const int* ijg,
+                            const double* di, const double* gg, const double* pr, double* x,
+                            double eps, int maxiter, int kpar, int n_edges_c, int n_nodes_c,
+                            const double* xyz, const int* nvkat, const int* nver,
+                            const double* sigma3d, const int* edges, const int* ig_t,
+                            const int* jg_t, const double* gg_t, const int* is_node_bound,
+                            mtcocr_stats* stats);
+
+/* Measurement hook: runs the SpMV kernel `reps` times on the handle's stream (precision 0 FP64,
+ * 1 FP32; nsys systems of the current batch layout, 1 for a block2x2 matrix) and returns the
+ * average kernel time from HIP events and the algorithmic bytes moved per launch. */
+int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, double* avg_ms,
+                      double* bytes_per_launch);
+
+/* Kernel-level hook for parity checks: y = op(x) on the current matrix, op 0 = SpMV (A x),
+ * op 1 = folded preconditioner (D_A^-1 x + G^T D_N^-1 G x), in precision 0 (FP64) or 1 (FP32
+ * storage, FP64 in/out). Uses system 0 of the current batch (omega = 1 for a block2x2 matrix).
+ * Counterparts: Block_2x2_solver::Mult_MV_block_2x2 (block_2x2_solver.cpp:73-173) and
+ * FoldedPreconditioner::ApplyPreconditionerComplex (FoldedPreconditioner.cpp:578-629). */
+int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, double* y);
+
+/* Matrix statistics of the device layout: complex / real non-zeros of the full matrix (explicit
+ * zeros dropped), number of interior nodes m of G, device bytes held by the handle. */
+int mtcocr_matrix_info(mtcocr_handle* h, int64_t* nnz_complex, int64_t* nnz_real, int64_t* m,
+                       int64_t* device_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTCOCR_H */

@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference's COCR_FP interface over the HIP C-ABI.
+
+`pcocr_2x2_folded` keeps the argument list, argument meaning and return codes of
+PCOCR::PCOCR_2x2_Folded (Code/COCR_FP/pcocr.h:27-31, pcocr.cpp:40-126) minus the per-thread
+scatter buffer y_omp: x is updated in place and the reference's request code is returned
+(0 REQ_OK, -1 REQ_X0_NULL, -2 REQ_DIV_BY_ZERO, -3 REQ_MAXITER, rci.h:114-133).
+
+`Solver` is the handle-based form (one matrix + gradient resident on the GPU, many right-hand
+sides, batched frequencies sharing K and S, mixed precision).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MtcocrStats, check, lib, ptr
+from .model import BlockSystem, MeshArrays, Portrait
+
+REQ_OK, REQ_X0_NULL, REQ_DIV_BY_ZERO, REQ_MAXITER = 0, -1, -2, -3
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One GPU-resident matrix (single system or K/S pair) plus the folded preconditioner's
+    gradient, solved for one or many right-hand sides."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        check(self._L.mtcocr_create(int(device), C.byref(h)), "mtcocr_create")
+        self._h = h
+        self.nb = 0
+        self.nsys = 0
+
+    def close(self):
+        if self._h:
+            self._L.mtcocr_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up -------------------------------------------------------------------------
+    def set_matrix(self, s: BlockSystem):
+        self._keep = (_i32(s.ig), _i32(s.jg), _i32(s.idi), _i32(s.ijg), _f64(s.di), _f64(s.gg))
+        check(self._L.mtcocr_set_matrix_block2x2(self._h, int(s.nb), *self._keep),
+              "mtcocr_set_matrix_block2x2")
+        self.nb = int(s.nb)
+
+    def set_matrix_ks(self, p: Portrait, kd, kg, sd, sg):
+        args = (_i32(p.ig), _i32(p.jg), _f64(kd), _f64(kg), _f64(sd), _f64(sg))
+        check(self._L.mtcocr_set_matrix_ks(self._h, int(p.nb), *args), "mtcocr_set_matrix_ks")
+        self.nb = int(p.nb)
+
+    def set_gradient(self, ma: MeshArrays):
+        check(self._L.mtcocr_set_gradient_mesh(
+            self._h, int(ma.kpar), int(len(ma.edges) // 2), int(ma.kuzlov), _i32(ma.nvkat),
+            _i32(ma.nver), _f64(ma.sig_flags), _i32(ma.edges), None, None, None,
+            _i32(ma.is_node_bound)), "mtcocr_set_gradient_mesh")
+
+    # ---- solves -------------------------------------------------------------------------
+    def solve(self, pr, x0=None, eps: float = 1e-8, maxiter: int = 10000,
+              precision: str = "fp64"):
+        """One system set by set_matrix. Returns (x, stats dict)."""
+        x = _f64(np.zeros(2 * self.nb) if x0 is None else x0).copy()
+        st = MtcocrStats()
+        rc = self._L.mtcocr_solve(self._h, _f64(pr), x, float(eps), int(maxiter),
+                                  PRECISION[precision], C.byref(st))
+        check(rc, "mtcocr_solve")
+        return x, st.as_dict()
+
+    def solve_batch(self, omega, prs, x0=None, eps: float = 1e-8, maxiter: int = 10000,
+                    precision: str = "fp64"):
+        """nsys systems A(omega_s) x_s = b_s on the K/S matrix. prs: (nsys, 2*nb)."""
+        prs = _f64(prs)
+        nsys = prs.shape[0]
+        x = _f64(np.zeros_like(prs) if x0 is None else x0).copy()
+        om = None if omega is None else _f64(omega)
+        sts = (MtcocrStats * nsys)()
+        rc = self._L.mtcocr_solve_batch(self._h, nsys, ptr(om), prs.ravel(), x.ravel(),
+                                        float(eps), int(maxiter), PRECISION[precision], sts)
+        check(rc, "mtcocr_solve_batch")
+        return x.reshape(prs.shape), [s.as_dict() for s in sts]
+
+    def set_rhs(self, prs, omega=None, x0=None):
+        prs = _f64(np.atleast_2d(prs))
+        self.nsys = prs.shape[0]
+        om = None if omega is None else _f64(omega)
+        x0 = None if x0 is None else _f64(x0)
+        check(self._L.mtcocr_set_rhs(self._h, self.nsys, ptr(om), prs.ravel(), ptr(x0)),
+              "mtcocr_set_rhs")
+
+    def solve_resident(self, eps: float = 1e-8, maxiter: int = 10000, precision: str = "fp64"):
+        sts = (MtcocrStats * self.nsys)()
+        rc = self._L.mtcocr_solve_resident(self._h, float(eps), int(maxiter),
+                                           PRECISION[precision], sts)
+        check(rc, "mtcocr_solve_resident")
+        return rc, [s.as_dict() for s in sts]
+
+    def get_solution(self):
+        x = np.empty(self.nsys * 2 * self.nb)
+        check(self._L.mtcocr_get_solution(self._h, x), "mtcocr_get_solution")
+        return x.reshape(self.nsys, 2 * self.nb)
+
+    # ---- measurement ----------------------------------------------------------------------
+    def bench_spmv(self, precision: str = "fp64", reps: int = 20):
+        ms, by = C.c_double(), C.c_double()
+        check(self._L.mtcocr_bench_spmv(self._h, PRECISION[precision], self.nsys, int(reps),
+                                        C.byref(ms), C.byref(by)), "mtcocr_bench_spmv")
+        return ms.value, by.value
+
+    def apply(self, op: str, x, precision: str = "fp64"):
+        """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""
+        y = np.empty(2 * self.nb)
+        check(self._L.mtcocr_apply(self._h, {"spmv": 0, "precond": 1}[op], PRECISION[precision],
+                                   _f64(x), y), "mtcocr_apply")
+        return y
+
+    def info(self) -> dict:
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(self._L.mtcocr_matrix_info(self._h, C.byref(a), C.byref(b), C.byref(c),
+                                         C.byref(d)), "mtcocr_matrix_info")
+        return {"nnz_complex": a.value, "nnz_real": b.value, "m": c.value,
+                "device_bytes": d.value}
+
+
+def pcocr_2x2_folded(n, ig, jg, idi, ijg, di, gg, pr, x, eps, maxiter, kpar, n_edges_c,
+                     n_nodes_c, xyz, nvkat, nver, sigma3d, edges, ig_t=None, jg_t=None,
+                     gg_t=None, is_node_bound=None, stats: dict | None = None) -> int:
+    """PCOCR::PCOCR_2x2_Folded on the GPU. x (float64, length n) is overwritten with the
+    solution; returns the reference's request code."""
+    if x.dtype != np.float64 or not x.flags.c_contiguous:
+        raise TypeError("x must be a C-contiguous float64 array (it is updated in place)")
+    st = MtcocrStats()
+    xyz_a = None if xyz is None else _f64(xyz)
+    rc = lib().mtcocr_pcocr_2x2_folded(
+        int(n), _i32(ig), _i32(jg), _i32(idi), _i32(ijg), _f64(di), _f64(gg), _f64(pr), x,
+        float(eps), int(maxiter), int(kpar), int(n_edges_c), int(n_nodes_c), ptr(xyz_a),
+        _i32(nvkat), _i32(nver), _f64(sigma3d), _i32(edges),
+        ptr(None if ig_t is None else _i32(ig_t)), ptr(None if jg_t is None else _i32(jg_t)),
+        ptr(None if gg_t is None else _f64(gg_t)), _i32(is_node_bound), C.byref(st))
+    check(rc, "mtcocr_pcocr_2x2_folded")
+    if stats is not None:
+        stats.update(st.as_dict())
+    return rc

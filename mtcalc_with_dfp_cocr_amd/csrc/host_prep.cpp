@@ -1,0 +1,269 @@
+// Host-side preparation of the solver's device layout (see host_prep.h).
+#include "host_prep.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "mtcocr.h"
+
+namespace mtcocr {
+
+namespace {
+
+struct Entry {
+  int col;
+  double a, b;
+  bool cplx;
+};
+
+// Row lists of the full matrix, built from lower-triangle blocks: row i receives its lower
+// blocks (ascending), its diagonal, then the transposed blocks of rows k > i (ascending k), so
+// columns are ascending within every row (the order of FoldedPreconditioner::From2x2ToCSR2x2_2,
+// Code/COCR_FP/FoldedPreconditioner.cpp:653-736).
+template <class GetLower, class GetDiag>
+int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag diag,
+               SlicedEll& out) {
+  if (nb <= 0 || !ig || !jg) return MTCOCR_ERR_ARG;
+  const int64_t nnzl = ig[nb];
+  std::vector<int64_t> cnt(nb + 1, 0);
+  for (int i = 0; i < nb; ++i) {
+    cnt[i] += 1 + (ig[i + 1] - ig[i]);
+    for (int64_t k = ig[i]; k < ig[i + 1]; ++k) {
+      const int j = jg[k];
+      if (j < 0 || j >= i) return MTCOCR_ERR_ARG;
+      cnt[j] += 1;
+    }
+  }
+  std::vector<int64_t> ptr(nb + 1, 0);
+  for (int i = 0; i < nb; ++i) ptr[i + 1] = ptr[i] + cnt[i];
+  std::vector<Entry> ent(ptr[nb]);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  out.diag_a.assign(nb, 0.0);
+  out.diag_b.assign(nb, 0.0);
+  for (int i = 0; i < nb; ++i) {
+    for (int64_t k = ig[i]; k < ig[i + 1]; ++k) {
+      Entry e = lower(k);
+      e.col = jg[k];
+      ent[fill[i]++] = e;
+    }
+    Entry d = diag(i);
+    d.col = i;
+    out.diag_a[i] = d.a;
+    out.diag_b[i] = d.b;
+    ent[fill[i]++] = d;
+  }
+  for (int i = 0; i < nb; ++i)
+    for (int64_t k = ig[i]; k < ig[i + 1]; ++k) {
+      Entry e = lower(k);
+      e.col = i;
+      ent[fill[jg[k]]++] = e;
+    }
+  (void)nnzl;
+  // slices of 64 rows: per slice the maximum complex / real counts (explicit zeros dropped,
+  // the diagonal always kept)
+  const int ns = (nb + 63) / 64;
+  out.nb = nb;
+  out.nslices = ns;
+  out.offc.assign(ns + 1, 0);
+  out.offr.assign(ns + 1, 0);
+  out.nc.assign(ns, 0);
+  out.nr.assign(ns, 0);
+  std::vector<int> rc(nb, 0), rr(nb, 0);
+  out.nnz_complex = out.nnz_real = 0;
+  for (int i = 0; i < nb; ++i) {
+    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const Entry& e = ent[k];
+      const bool keep = (e.col == i) || e.a != 0.0 || e.b != 0.0;
+      if (!keep) continue;
+      if (e.cplx) ++rc[i]; else ++rr[i];
+    }
+    out.nnz_complex += rc[i];
+    out.nnz_real += rr[i];
+    const int s = i >> 6;
+    out.nc[s] = std::max(out.nc[s], rc[i]);
+    out.nr[s] = std::max(out.nr[s], rr[i]);
+  }
+  for (int s = 0; s < ns; ++s) {
+    const int64_t oc = (int64_t)out.offc[s] + 64LL * out.nc[s];
+    const int64_t orr = (int64_t)out.offr[s] + 64LL * out.nr[s];
+    if (oc > 2147483000LL || orr > 2147483000LL) return MTCOCR_ERR_ARG;
+    out.offc[s + 1] = int(oc);
+    out.offr[s + 1] = int(orr);
+  }
+  const int64_t totc = out.offc[ns], totr = out.offr[ns];
+  out.colc.assign(totc, 0);
+  out.ac.assign(totc, 0.0);
+  out.bc.assign(totc, 0.0);
+  out.colr.assign(totr, 0);
+  out.ar.assign(totr, 0.0);
+  for (int i = 0; i < nb; ++i) {
+    const int s = i >> 6, lane = i & 63;
+    int kc = 0, kr = 0;
+    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const Entry& e = ent[k];
+      const bool keep = (e.col == i) || e.a != 0.0 || e.b != 0.0;
+      if (!keep) continue;
+      if (e.cplx) {
+        const int64_t p = out.offc[s] + 64LL * kc + lane;
+        out.colc[p] = e.col;
+        out.ac[p] = e.a;
+        out.bc[p] = e.b;
+        ++kc;
+      } else {
+        const int64_t p = out.offr[s] + 64LL * kr + lane;
+        out.colr[p] = e.col;
+        out.ar[p] = e.a;
+        ++kr;
+      }
+    }
+    // padding slots point at the row itself with zero value
+    for (; kc < out.nc[s]; ++kc) out.colc[out.offc[s] + 64LL * kc + lane] = i;
+    for (; kr < out.nr[s]; ++kr) out.colr[out.offr[s] + 64LL * kr + lane] = i;
+  }
+  // lanes past nb in the last slice keep column 0 and zero values
+  return 0;
+}
+
+}  // namespace
+
+int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, const int* ijg,
+                      const double* di, const double* gg, SlicedEll& out) {
+  if (!idi || !ijg || !di || !gg) return MTCOCR_ERR_ARG;
+  auto lower = [&](int64_t k) {
+    Entry e;
+    const int beg = ijg[k], sz = ijg[k + 1] - ijg[k];
+    e.a = gg[beg];
+    e.b = (sz == 2) ? gg[beg + 1] : 0.0;
+    e.cplx = (e.b != 0.0);
+    e.col = 0;
+    return e;
+  };
+  auto diag = [&](int i) {
+    Entry e;
+    const int beg = idi[i], sz = idi[i + 1] - idi[i];
+    e.a = di[beg];
+    e.b = (sz == 2) ? di[beg + 1] : 0.0;
+    e.cplx = (e.b != 0.0);
+    e.col = i;
+    return e;
+  };
+  return build_rows(nb, ig, jg, lower, diag, out);
+}
+
+int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const double* k_gg,
+                const double* s_di, const double* s_gg, SlicedEll& out) {
+  if (!k_di || !k_gg || !s_di || !s_gg) return MTCOCR_ERR_ARG;
+  auto lower = [&](int64_t k) {
+    Entry e;
+    e.a = k_gg[k];
+    e.b = s_gg[k];
+    e.cplx = (e.b != 0.0);
+    e.col = 0;
+    return e;
+  };
+  auto diag = [&](int i) {
+    Entry e;
+    e.a = k_di[i];
+    e.b = s_di[i];
+    e.cplx = (e.b != 0.0);
+    e.col = i;
+    return e;
+  };
+  return build_rows(nb, ig, jg, lower, diag, out);
+}
+
+int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
+                   const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
+                   const double* gg_t, const int* is_node_bound, GradientEll& out) {
+  if (kpar <= 0 || n_edges <= 0 || n_nodes_c <= 0 || !nvkat || !nver || !sigma3d || !edges ||
+      !is_node_bound)
+    return MTCOCR_ERR_ARG;
+  // nodes of conductive cells, boundary nodes excluded, renumbered in ascending order
+  std::vector<char> in_sigma(n_nodes_c, 0);
+  for (int i = 0; i < kpar; ++i) {
+    if (!(sigma3d[nvkat[i]] > 0.0)) continue;
+    for (int j = 0; j < 8; ++j) {
+      const int node = nver[(int64_t)i * 14 + j];
+      if (node < 0) return MTCOCR_ERR_ARG;
+      if (node < n_nodes_c) {
+        if (is_node_bound[node]) continue;
+        in_sigma[node] = 1;
+      }
+    }
+  }
+  std::vector<int> renum(n_nodes_c, -1);
+  int m = 0;
+  for (int i = 0; i < n_nodes_c; ++i)
+    if (in_sigma[i]) renum[i] = m++;
+  // G^T rows (edge -> nodes): -1/2 at the first node, +1/2 at the second; hanging nodes
+  // (index >= n_nodes_c) spread through the T-matrix ig_t/jg_t/gg_t
+  std::vector<int64_t> tptr(n_edges + 1, 0);
+  std::vector<int> tcol;
+  std::vector<double> tval;
+  tcol.reserve((size_t)n_edges * 2);
+  tval.reserve((size_t)n_edges * 2);
+  for (int i = 0; i < n_edges; ++i) {
+    for (int k = 0; k < 2; ++k) {
+      const int v = edges[(int64_t)i * 2 + k];
+      const double sgn = (k == 0) ? -0.5 : 0.5;
+      if (v < n_nodes_c) {
+        if (v >= 0 && in_sigma[v]) {
+          tcol.push_back(renum[v]);
+          tval.push_back(sgn);
+        }
+      } else {
+        if (!ig_t || !jg_t || !gg_t) return MTCOCR_ERR_ARG;
+        for (int ii = ig_t[v]; ii < ig_t[v + 1]; ++ii) {
+          const int kk = jg_t[ii];
+          if (in_sigma[kk]) {
+            tcol.push_back(renum[kk]);
+            tval.push_back(sgn * gg_t[ii]);
+          }
+        }
+      }
+    }
+    tptr[i + 1] = (int64_t)tcol.size();
+  }
+  // G rows (node -> edges, ascending edge)
+  std::vector<int64_t> gcnt(m + 1, 0);
+  for (size_t k = 0; k < tcol.size(); ++k) gcnt[tcol[k]]++;
+  std::vector<int64_t> gptr(m + 1, 0);
+  for (int i = 0; i < m; ++i) gptr[i + 1] = gptr[i] + gcnt[i];
+  std::vector<int> gc(tcol.size());
+  std::vector<double> gv(tcol.size());
+  std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+  for (int e = 0; e < n_edges; ++e)
+    for (int64_t k = tptr[e]; k < tptr[e + 1]; ++k) {
+      const int node = tcol[k];
+      gc[fill[node]] = e;
+      gv[fill[node]] = tval[k];
+      fill[node]++;
+    }
+  out.n = n_edges;
+  out.m = m;
+  out.wg = 0;
+  out.wgt = 0;
+  for (int i = 0; i < m; ++i) out.wg = std::max<int>(out.wg, int(gptr[i + 1] - gptr[i]));
+  for (int e = 0; e < n_edges; ++e) out.wgt = std::max<int>(out.wgt, int(tptr[e + 1] - tptr[e]));
+  out.gcol.assign((size_t)out.wg * m, 0);
+  out.gval.assign((size_t)out.wg * m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    int k = 0;
+    for (int64_t q = gptr[i]; q < gptr[i + 1]; ++q, ++k) {
+      out.gcol[(size_t)k * m + i] = gc[q];
+      out.gval[(size_t)k * m + i] = gv[q];
+    }
+  }
+  out.gtcol.assign((size_t)out.wgt * n_edges, 0);
+  out.gtval.assign((size_t)out.wgt * n_edges, 0.0);
+  for (int e = 0; e < n_edges; ++e) {
+    int k = 0;
+    for (int64_t q = tptr[e]; q < tptr[e + 1]; ++q, ++k) {
+      out.gtcol[(size_t)k * n_edges + e] = tcol[q];
+      out.gtval[(size_t)k * n_edges + e] = tval[q];
+    }
+  }
+  return 0;
+}
+
+}  // namespace mtcocr

@@ -1,0 +1,53 @@
+// Host-side preparation of the solver's device layout.
+//
+// SlicedEll: the full complex-symmetric matrix (both triangles, explicit zero blocks dropped)
+// in 64-row slices. Each slice stores its rows' entries column-major (slot k of row r at
+// off + k*64 + (r % 64)) so a wavefront's 64 lanes read 64 consecutive entries. Entries are
+// split into a "complex" section (col, a, b) meaning a + i*omega*b, and a "real" section
+// (col, a). A single system from the reference's files uses omega = 1 (a = Re, b = Im); a
+// frequency-independent pair uses a = K, b = S and A(omega) = K + i*omega*S.
+//
+// GradientEll: the folded preconditioner's discrete gradient G (interior nodes x edges) and its
+// transpose, padded row-wise (column-major) to their maximum row length.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace mtcocr {
+
+struct SlicedEll {
+  int nb = 0;
+  int nslices = 0;
+  std::vector<int> offc, nc, offr, nr;  // per slice: entry offset and slot count
+  std::vector<int> colc, colr;
+  std::vector<double> ac, bc, ar;
+  std::vector<double> diag_a, diag_b;   // diagonal as a + i*omega*b
+  int64_t nnz_complex = 0, nnz_real = 0;  // stored (unpadded) entries
+};
+
+struct GradientEll {
+  int n = 0;   // edges (rows of G^T)
+  int m = 0;   // interior nodes (rows of G)
+  int wg = 0;  // max entries per G row
+  int wgt = 0; // max entries per G^T row
+  std::vector<int> gcol;     // [k * m + i]
+  std::vector<double> gval;
+  std::vector<int> gtcol;    // [k * n + e]
+  std::vector<double> gtval;
+};
+
+// From the reference's lower-triangle 2x2-block arrays (0-based). Returns 0 or a negative code.
+int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, const int* ijg,
+                      const double* di, const double* gg, SlicedEll& out);
+
+// From a K/S pair, one real per block on the same portrait.
+int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const double* k_gg,
+                const double* s_di, const double* s_gg, SlicedEll& out);
+
+// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259).
+int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
+                   const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
+                   const double* gg_t, const int* is_node_bound, GradientEll& out);
+
+}  // namespace mtcocr

@@ -1,0 +1,1624 @@
+// MI355X COCR solver for the complex-symmetric edge-element system of the MT forward problem.
+//
+// Method (the reference's Code/COCR_FP, restated for the GPU):
+//   COCR request order   Code/COCR_FP/pcocr_rci.cpp:50-181 (stages 1-8)
+//   stop test            Code/COCR_FP/pcocr.cpp:91-96, rci.cpp:66-90 (MRS residual / ||b||)
+//   MRS smoothing        Code/COCR_FP/MRS.cpp:85-153 (real-valued, over the 2*nb reals)
+//   preconditioner       Code/COCR_FP/FoldedPreconditioner.cpp:473-629 (diag(A)^-1 +
+//                        G^T diag(G A G^T)^-1 G)
+//   SpMV                 Code/COCR_FP/block_2x2_solver.cpp:73-173 (A = L + L^T + D)
+//   dot products         Code/COCR_FP/base_solver.cpp:382-399 (unconjugated)
+//
+// Device design (one GPU, HIP streams and graphs):
+//   * S systems share one sparsity; vectors are laid out [row][system] so that a gather of
+//     x[col] for all systems is one contiguous segment. A single system is S = 1.
+//   * The matrix is a sliced ELL of 64-row slices (host_prep.h), complex entries a + i*omega*b,
+//     read once per row group and reused by every system of the batch.
+//   * Every reduction is deterministic: fixed grid, per-block partials in a fixed order, then a
+//     one-block "scalar" kernel that also carries the solver's scalar logic (alpha, beta, MRS
+//     weight, stop test, iteration count). No atomics, no inter-block fences.
+//   * Per-system status flags retire systems independently; the host polls a running count
+//     every chunk of iterations, and a chunk is replayed from a captured hipGraph.
+//   * Mixed precision: complex FP32 inner COCR (SpMV, preconditioner, updates), FP64 outer
+//     iterative refinement (true residual, update, final residual).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "host_prep.h"
+#include "mtcocr.h"
+
+namespace mtcocr {
+
+// ------------------------------------------------------------------------------------------
+// small helpers
+
+constexpr int kBlock = 256;      // threads per block of the streaming kernels
+constexpr int kMaxBlocks = 1024; // cap of the grid-stride kernels (fixed reduction grid)
+constexpr int kScalarThreads = 1024;
+constexpr int kXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+enum : int { ST_RUNNING = 1, ST_INACTIVE = 2 };
+enum : int { MRS_SKIP = 0, MRS_COPY = 1, MRS_BLEND = 2 };
+
+template <class T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+// Per-system scalar state (double precision in every mode).
+struct SysScalars {
+  double zs_re, zs_im, zs1_re, zs1_im;
+  double aw_re, aw_im, al_re, al_im, be_re, be_im;
+  double r_old, res, rel, w, eps;
+  double scale, prev_rel, b_norm;
+  int mode, status, iter, total_iter;
+  int outer_status, outer, pad0, pad1;
+};
+
+template <class T>
+struct EllView {
+  const int *offc, *nc, *offr, *nr, *colc, *colr;
+  const T *ac, *bc, *ar;
+  int nb;
+};
+
+template <class T>
+struct GView {
+  const int *gcol, *gtcol;
+  const T *gval, *gtval;
+  int m, n, wg, wgt;
+};
+
+// complex division with scaling (Smith), used for alpha, beta and reciprocals
+__host__ __device__ inline void cdiv(double ar, double ai, double br, double bi, double& qr,
+                                     double& qi) {
+  if (fabs(br) >= fabs(bi)) {
+    const double r = bi / br, d = br + bi * r;
+    qr = (ar + ai * r) / d;
+    qi = (ai - ar * r) / d;
+  } else {
+    const double r = br / bi, d = bi + br * r;
+    qr = (ar * r + ai) / d;
+    qi = (ai * r - ar) / d;
+  }
+}
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+inline int grid_for(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > kMaxBlocks) b = kMaxBlocks;
+  if (b < kXcd) b = kXcd;
+  return round_up(int(b), kXcd);
+}
+
+// XCD-aware tile walk: workgroup b runs on XCD b % 8; each XCD owns a contiguous range of
+// 256-element tiles so neighbouring rows (and their x gathers) share one L2.
+struct TileWalk {
+  long long first, last, step;
+  __device__ TileWalk(long long ntiles) {
+    const int nblk = gridDim.x;
+    const int xcd = blockIdx.x % kXcd;
+    const int lb = blockIdx.x / kXcd;
+    const int per = nblk / kXcd;
+    const long long lo = ntiles * xcd / kXcd;
+    const long long hi = ntiles * (xcd + 1) / kXcd;
+    first = lo + lb;
+    last = hi;
+    step = per;
+  }
+};
+
+// Block reduction of NV values per system; lanes with equal threadIdx % SP share a system.
+// Writes partials[blockIdx.x * (SP*NV) + s*NV + k].
+template <int SP, int NV>
+__device__ inline void block_partials(double (&v)[NV], double* partials) {
+  for (int off = SP; off < 64; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  __shared__ double sh[kBlock / 64][64 * NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < SP)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wave][lane * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < SP * NV) {
+    double s = 0.0;
+    for (int wv = 0; wv < kBlock / 64; ++wv) s += sh[wv][threadIdx.x];
+    partials[(size_t)blockIdx.x * (SP * NV) + threadIdx.x] = s;
+  }
+}
+
+// One-block reduction of nblk x P partials into tot[P] (fixed order).
+template <int P>
+__device__ inline void reduce_partials(const double* part, int nblk, double* tot) {
+  constexpr int G = kScalarThreads / P;
+  __shared__ double tmp[kScalarThreads];
+  const int t = threadIdx.x;
+  const int q = t % P, j = t / P;
+  double acc = 0.0;
+  if (j < G)
+    for (int b = j; b < nblk; b += G) acc += part[(size_t)b * P + q];
+  tmp[t] = acc;
+  __syncthreads();
+  if (t < P) {
+    double s = 0.0;
+    for (int jj = 0; jj < G; ++jj) s += tmp[jj * P + t];
+    tot[t] = s;
+  }
+  __syncthreads();
+}
+
+__device__ inline void count_running(const SysScalars* sc, int nsp, int* nrun) {
+  // called by thread 0 after the per-system updates
+  int c = 0;
+  for (int s = 0; s < nsp; ++s) c += (sc[s].status == ST_RUNNING);
+  *nrun = c;
+}
+
+// ------------------------------------------------------------------------------------------
+// element-wise complex helpers: z = x + a*y written exactly as Base_solver::Cmplx_axpy
+
+template <class T2>
+__device__ inline T2 caxpy(T2 x, T2 y, double ar, double ai) {
+  using T = decltype(x.x);
+  const T a_re = T(ar), a_im = T(ai);
+  T2 r;
+  r.x = x.x + (y.x * a_re - y.y * a_im);
+  r.y = x.y + (y.x * a_im + y.y * a_re);
+  return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// kernels
+
+template <class T2>
+__global__ void k_fill(T2* v, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = i; k < n; k += stride) {
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+    v[k] = z;
+  }
+}
+
+// D_A^-1 from the FP64 diagonal: 1 / (a + i*omega*b) (FoldedPreconditioner.cpp:507-520)
+template <class T, int SP>
+__global__ void k_diag_a(const double* da, const double* db, const double* omega,
+                         typename Vec2<T>::type* out, int nb) {
+  const long long n = (long long)nb * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long row = t / SP;
+    const int s = int(t % SP);
+    double qr, qi;
+    cdiv(1.0, 0.0, da[row], omega[s] * db[row], qr, qi);
+    out[t].x = T(qr);
+    out[t].y = T(qi);
+  }
+}
+
+// D_N^-1 = 1 / (G A G^T)_ii from the FP64 matrix (FoldedPreconditioner.cpp:527-568): for every
+// edge j of node i, row j of A is scanned for the node's other edges.
+template <class T, int SP>
+__global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega,
+                         typename Vec2<T>::type* out) {
+  const long long n = (long long)G.m * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = int(t / SP);
+    const int s = int(t % SP);
+    const double w = omega[s];
+    int ecol[8];
+    double eval[8];
+    const int wg = G.wg < 8 ? G.wg : 8;
+    for (int k = 0; k < wg; ++k) {
+      ecol[k] = G.gcol[(size_t)k * G.m + i];
+      eval[k] = G.gval[(size_t)k * G.m + i];
+    }
+    double dr = 0.0, di = 0.0;
+    for (int kj = 0; kj < wg; ++kj) {
+      const double gj = eval[kj];
+      if (gj == 0.0) continue;
+      const int j = ecol[kj];
+      const int slice = j >> 6, lane = j & 63;
+      double sr = 0.0, si = 0.0;
+      const int oc = A.offc[slice], ncs = A.nc[slice];
+      for (int q = 0; q < ncs; ++q) {
+        const int idx = oc + q * 64 + lane;
+        const int c = A.colc[idx];
+        for (int kk = 0; kk < wg; ++kk)
+          if (ecol[kk] == c && eval[kk] != 0.0) {
+            sr += A.ac[idx] * eval[kk];
+            si += w * A.bc[idx] * eval[kk];
+          }
+      }
+      const int orr = A.offr[slice], nrs = A.nr[slice];
+      for (int q = 0; q < nrs; ++q) {
+        const int idx = orr + q * 64 + lane;
+        const int c = A.colr[idx];
+        for (int kk = 0; kk < wg; ++kk)
+          if (ecol[kk] == c && eval[kk] != 0.0) sr += A.ar[idx] * eval[kk];
+      }
+      dr += sr * gj;
+      di += si * gj;
+    }
+    double qr, qi;
+    cdiv(1.0, 0.0, dr, di, qr, qi);
+    out[t].x = T(qr);
+    out[t].y = T(qi);
+  }
+}
+
+// y = A x (MODE 0, partial (y, x) unconjugated) or r = b - A x (MODE 1, partials ||b||^2 and
+// ||r||^2, plus the MRS initialisation ym = x, sm = r of MRS.cpp:91-97). MODE 2 is MODE 1 gated
+// by the outer (refinement) status instead of the COCR status.
+template <class T, int SP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv(EllView<T> A, const double* omega,
+                                                 const typename Vec2<T>::type* __restrict__ x,
+                                                 typename Vec2<T>::type* __restrict__ y,
+                                                 const typename Vec2<T>::type* __restrict__ b,
+                                                 typename Vec2<T>::type* __restrict__ ym,
+                                                 typename Vec2<T>::type* __restrict__ sm,
+                                                 const SysScalars* __restrict__ sc,
+                                                 const int* __restrict__ nrun, double* partials) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)A.nb * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  double v[2] = {0.0, 0.0};
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if ((MODE == 2 ? sc[s].outer_status : sc[s].status) != ST_RUNNING) continue;
+    const int row = int(t / SP);
+    const int slice = row >> 6, lane = row & 63;
+    const T w = T(omega[s]);
+    T yr = 0, yi = 0;
+    {
+      const int oc = A.offc[slice], ncs = A.nc[slice];
+#pragma unroll 4
+      for (int k = 0; k < ncs; ++k) {
+        const int idx = oc + k * 64 + lane;
+        const int c = A.colc[idx];
+        const T a = A.ac[idx];
+        const T bb = A.bc[idx] * w;
+        const T2 xv = x[(size_t)c * SP + s];
+        yr += a * xv.x - bb * xv.y;
+        yi += a * xv.y + bb * xv.x;
+      }
+    }
+    {
+      const int orr = A.offr[slice], nrs = A.nr[slice];
+#pragma unroll 4
+      for (int k = 0; k < nrs; ++k) {
+        const int idx = orr + k * 64 + lane;
+        const int c = A.colr[idx];
+        const T a = A.ar[idx];
+        const T2 xv = x[(size_t)c * SP + s];
+        yr += a * xv.x;
+        yi += a * xv.y;
+      }
+    }
+    if (MODE == 0) {
+      T2 o;
+      o.x = yr;
+      o.y = yi;
+      y[t] = o;
+      const T2 xs = x[t];
+      v[0] += double(yr) * double(xs.x) - double(yi) * double(xs.y);
+      v[1] += double(yr) * double(xs.y) + double(yi) * double(xs.x);
+    } else {
+      const T2 bv = b[t];
+      T2 o;
+      o.x = bv.x - yr;
+      o.y = bv.y - yi;
+      y[t] = o;
+      v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
+      v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
+      if (ym) {
+        ym[t] = x[t];
+        sm[t] = o;
+      }
+    }
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// help = D_N^-1 (G v), v = z + beta*a (USE_BETA) or v = z  (FoldedPreconditioner.cpp:598-615)
+template <class T, int SP, bool USE_BETA>
+__global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
+                                                    const typename Vec2<T>::type* __restrict__ z,
+                                                    const typename Vec2<T>::type* __restrict__ a,
+                                                    const typename Vec2<T>::type* __restrict__ dn,
+                                                    typename Vec2<T>::type* __restrict__ help,
+                                                    const SysScalars* __restrict__ sc,
+                                                    const int* __restrict__ nrun) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)G.m * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if (sc[s].status != ST_RUNNING) continue;
+    const int i = int(t / SP);
+    const double br = USE_BETA ? sc[s].be_re : 0.0, bi = USE_BETA ? sc[s].be_im : 0.0;
+    T hr = 0, hi = 0;
+    for (int k = 0; k < G.wg; ++k) {
+      const int e = G.gcol[(size_t)k * G.m + i];
+      const T g = G.gval[(size_t)k * G.m + i];
+      T2 vv = z[(size_t)e * SP + s];
+      if (USE_BETA) vv = caxpy(vv, a[(size_t)e * SP + s], br, bi);
+      hr += g * vv.x;
+      hi += g * vv.y;
+    }
+    const T2 d = dn[t];
+    T2 o;
+    o.x = hr * d.x - hi * d.y;
+    o.y = hi * d.x + hr * d.y;
+    help[t] = o;
+  }
+}
+
+// Prologue form: out = D_A^-1 v + G^T help (FoldedPreconditioner.cpp:585-589,618-627)
+template <class T, int SP>
+__global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
+    GView<T> G, const typename Vec2<T>::type* __restrict__ v,
+    const typename Vec2<T>::type* __restrict__ da, const typename Vec2<T>::type* __restrict__ help,
+    typename Vec2<T>::type* __restrict__ out, const SysScalars* __restrict__ sc,
+    const int* __restrict__ nrun) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)G.n * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if (sc[s].status != ST_RUNNING) continue;
+    const int e = int(t / SP);
+    const T2 xv = v[t], d = da[t];
+    T yr = xv.x * d.x - xv.y * d.y;
+    T yi = xv.x * d.y + xv.y * d.x;
+    for (int k = 0; k < G.wgt; ++k) {
+      const int node = G.gtcol[(size_t)k * G.n + e];
+      const T g = G.gtval[(size_t)k * G.n + e];
+      const T2 h = help[(size_t)node * SP + s];
+      yr += g * h.x;
+      yi += g * h.y;
+    }
+    T2 o;
+    o.x = yr;
+    o.y = yi;
+    out[t] = o;
+  }
+}
+
+// Iteration form (stages 5-6): a = z + beta*a, p = s + beta*p, w = M^-1 a, partial (a, w).
+template <class T, int SP>
+__global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
+    GView<T> G, const typename Vec2<T>::type* __restrict__ z, typename Vec2<T>::type* __restrict__ a,
+    const typename Vec2<T>::type* __restrict__ s_vec, typename Vec2<T>::type* __restrict__ p,
+    typename Vec2<T>::type* __restrict__ w, const typename Vec2<T>::type* __restrict__ da,
+    const typename Vec2<T>::type* __restrict__ help, const SysScalars* __restrict__ sc,
+    const int* __restrict__ nrun, double* partials) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)G.n * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  double v[2] = {0.0, 0.0};
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if (sc[s].status != ST_RUNNING) continue;
+    const int e = int(t / SP);
+    const double br = sc[s].be_re, bi = sc[s].be_im;
+    const T2 an = caxpy(z[t], a[t], br, bi);
+    const T2 pn = caxpy(s_vec[t], p[t], br, bi);
+    const T2 d = da[t];
+    T yr = an.x * d.x - an.y * d.y;
+    T yi = an.x * d.y + an.y * d.x;
+    for (int k = 0; k < G.wgt; ++k) {
+      const int node = G.gtcol[(size_t)k * G.n + e];
+      const T g = G.gtval[(size_t)k * G.n + e];
+      const T2 h = help[(size_t)node * SP + s];
+      yr += g * h.x;
+      yi += g * h.y;
+    }
+    T2 o;
+    o.x = yr;
+    o.y = yi;
+    a[t] = an;
+    p[t] = pn;
+    w[t] = o;
+    v[0] += double(an.x) * double(yr) - double(an.y) * double(yi);
+    v[1] += double(an.x) * double(yi) + double(an.y) * double(yr);
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// Stage 6 tail + MRS first pass: x += alpha p, r -= alpha a, u = r - sm, partials (sm,u),(u,u)
+template <class T, int SP>
+__global__ __launch_bounds__(kBlock) void k_xr_mrs1(
+    typename Vec2<T>::type* __restrict__ x, const typename Vec2<T>::type* __restrict__ p,
+    typename Vec2<T>::type* __restrict__ r, const typename Vec2<T>::type* __restrict__ a,
+    const typename Vec2<T>::type* __restrict__ sm, const SysScalars* __restrict__ sc,
+    const int* __restrict__ nrun, double* partials, int nb) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)nb * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  double v[2] = {0.0, 0.0};
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if (sc[s].status != ST_RUNNING) continue;
+    const double ar = sc[s].al_re, ai = sc[s].al_im;
+    const T2 xn = caxpy(x[t], p[t], ar, ai);
+    const T2 rn = caxpy(r[t], a[t], -ar, -ai);
+    const T2 smv = sm[t];
+    const T ur = rn.x - smv.x, ui = rn.y - smv.y;
+    x[t] = xn;
+    r[t] = rn;
+    v[0] += double(smv.x) * double(ur) + double(smv.y) * double(ui);
+    v[1] += double(ur) * double(ur) + double(ui) * double(ui);
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// MRS second pass (MRS.cpp:113-148) and stage 7's s -= alpha w; partial ||sm||^2.
+template <class T, int SP>
+__global__ __launch_bounds__(kBlock) void k_mrs2(
+    const typename Vec2<T>::type* __restrict__ x, const typename Vec2<T>::type* __restrict__ r,
+    typename Vec2<T>::type* __restrict__ ym, typename Vec2<T>::type* __restrict__ sm,
+    typename Vec2<T>::type* __restrict__ s_vec, const typename Vec2<T>::type* __restrict__ w,
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, double* partials, int nb) {
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)nb * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  double v[1] = {0.0};
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    const int s = int(t % SP);
+    if (sc[s].status != ST_RUNNING) continue;
+    const int mode = sc[s].mode;
+    if (mode == MRS_BLEND) {
+      const T wm = T(sc[s].w);
+      const T2 xv = x[t], rv = r[t];
+      T2 yv = ym[t], sv = sm[t];
+      const T ur = rv.x - sv.x, ui = rv.y - sv.y;
+      yv.x += wm * (xv.x - yv.x);
+      yv.y += wm * (xv.y - yv.y);
+      sv.x += wm * ur;
+      sv.y += wm * ui;
+      ym[t] = yv;
+      sm[t] = sv;
+      v[0] += double(sv.x) * double(sv.x) + double(sv.y) * double(sv.y);
+    } else if (mode == MRS_COPY) {
+      const T2 rv = r[t];
+      ym[t] = x[t];
+      sm[t] = rv;
+      v[0] += double(rv.x) * double(rv.x) + double(rv.y) * double(rv.y);
+    }
+    const double ar = sc[s].al_re, ai = sc[s].al_im;
+    s_vec[t] = caxpy(s_vec[t], w[t], -ar, -ai);
+  }
+  block_partials<SP, 1>(v, partials);
+}
+
+// ---- scalar kernels (one block) ----------------------------------------------------------
+
+// after the initial residual: r_old = ||b||, X0 test, MRS residual, beta = 0
+template <int SP>
+__global__ __launch_bounds__(kScalarThreads) void s_prologue(const double* part, int nblk,
+                                                             SysScalars* sc, int* nrun) {
+  __shared__ double tot[SP * 2];
+  reduce_partials<SP * 2>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].status == ST_RUNNING) {
+    const double bb = tot[s * 2], rr = tot[s * 2 + 1];
+    sc[s].r_old = bb > 0.0 ? sqrt(bb) : 0.0;
+    sc[s].b_norm = sc[s].r_old;
+    const double rn = rr > 0.0 ? sqrt(rr) : 0.0;
+    sc[s].res = rn;  // MRS init: residual = ||r||
+    sc[s].rel = 1.0;
+    sc[s].be_re = sc[s].be_im = 0.0;
+    sc[s].iter = 0;
+    if (rn < 1e-30) sc[s].status = MTCOCR_X0_NULL;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) count_running(sc, SP, nrun);
+}
+
+// after k_pc_edge_iter: aw, zero test, alpha = zs / aw (pcocr_rci.cpp:288-299)
+template <int SP>
+__global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, int nblk,
+                                                          SysScalars* sc, int* nrun) {
+  __shared__ double tot[SP * 2];
+  reduce_partials<SP * 2>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].status == ST_RUNNING) {
+    const double awr = tot[s * 2], awi = tot[s * 2 + 1];
+    sc[s].aw_re = awr;
+    sc[s].aw_im = awi;
+    if (fabs(awr) < 1e-40 && fabs(awi) < 1e-40) {
+      sc[s].status = MTCOCR_DIV_BY_ZERO;
+    } else {
+      double qr, qi;
+      cdiv(sc[s].zs_re, sc[s].zs_im, awr, awi, qr, qi);
+      sc[s].al_re = qr;
+      sc[s].al_im = qi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) count_running(sc, SP, nrun);
+}
+
+// after k_xr_mrs1: MRS weight w = -(s,u)/(u,u) clamped (MRS.cpp:113-121)
+template <int SP>
+__global__ __launch_bounds__(kScalarThreads) void s_mrs(const double* part, int nblk,
+                                                        SysScalars* sc) {
+  __shared__ double tot[SP * 2];
+  reduce_partials<SP * 2>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].status == ST_RUNNING) {
+    const double su = tot[s * 2], uu = tot[s * 2 + 1];
+    double w = (fabs(uu) < 1e-100) ? 0.0 : -su / uu;
+    int mode;
+    if (w < 0) {
+      w = 0;
+      mode = MRS_SKIP;
+    } else if (w > 1) {
+      w = 1;
+      mode = MRS_COPY;
+    } else {
+      mode = MRS_BLEND;
+    }
+    sc[s].w = w;
+    sc[s].mode = mode;
+  }
+}
+
+// after k_mrs2: residual and stop test (rci.cpp:66-77)
+template <int SP>
+__global__ __launch_bounds__(kScalarThreads) void s_stop(const double* part, int nblk,
+                                                         SysScalars* sc, int* nrun) {
+  __shared__ double tot[SP];
+  reduce_partials<SP>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].status == ST_RUNNING) {
+    const double ss = tot[s];
+    if (sc[s].mode == MRS_BLEND) sc[s].res = sqrt(ss);
+    else if (sc[s].mode == MRS_COPY) sc[s].res = ss > 0.0 ? sqrt(ss) : 0.0;
+    sc[s].rel = sc[s].res / sc[s].r_old;
+    if (sc[s].rel < sc[s].eps) sc[s].status = MTCOCR_OK;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) count_running(sc, SP, nrun);
+}
+
+// after k_spmv (z = A s): zs, beta, next iteration (pcocr_rci.cpp:263-270,323-337)
+template <int SP, bool FIRST>
+__global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int nblk,
+                                                         SysScalars* sc, int* nrun, int maxiter) {
+  __shared__ double tot[SP * 2];
+  reduce_partials<SP * 2>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].status == ST_RUNNING) {
+    const double zr = tot[s * 2], zi = tot[s * 2 + 1];
+    if (FIRST) {
+      sc[s].zs_re = zr;
+      sc[s].zs_im = zi;
+      sc[s].be_re = sc[s].be_im = 0.0;
+    } else {
+      sc[s].zs1_re = sc[s].zs_re;
+      sc[s].zs1_im = sc[s].zs_im;
+      sc[s].zs_re = zr;
+      sc[s].zs_im = zi;
+      // the reference tests aw here, not zs (pcocr_rci.cpp:328)
+      if (fabs(sc[s].aw_re) < 1e-40 && fabs(sc[s].aw_im) < 1e-40) {
+        sc[s].status = MTCOCR_DIV_BY_ZERO;
+      } else {
+        double qr, qi;
+        cdiv(zr, zi, sc[s].zs1_re, sc[s].zs1_im, qr, qi);
+        sc[s].be_re = qr;
+        sc[s].be_im = qi;
+      }
+    }
+    if (sc[s].status == ST_RUNNING) {
+      sc[s].iter += 1;
+      if (sc[s].iter > maxiter) sc[s].status = MTCOCR_MAXITER;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) count_running(sc, SP, nrun);
+}
+
+// ---- mixed precision: FP64 outer refinement ------------------------------------------------
+
+// after the FP64 true residual: relative residual, convergence / stall, inner set-up
+template <int SP>
+__global__ __launch_bounds__(kScalarThreads) void s_outer(const double* part, int nblk,
+                                                          SysScalars* sc, int* nrun_outer,
+                                                          int* nrun_inner, double eps,
+                                                          double inner_floor, int first,
+                                                          int max_outer) {
+  __shared__ double tot[SP * 2];
+  reduce_partials<SP * 2>(part, nblk, tot);
+  const int s = threadIdx.x;
+  if (s < SP) {
+    if (sc[s].outer_status == ST_RUNNING) {
+      const double bb = tot[s * 2], rr = tot[s * 2 + 1];
+      const double rn = rr > 0.0 ? sqrt(rr) : 0.0;
+      if (first) {
+        sc[s].b_norm = bb > 0.0 ? sqrt(bb) : 0.0;
+        sc[s].prev_rel = 1e300;
+        sc[s].outer = 0;
+        sc[s].total_iter = 0;
+      }
+      const double bn = sc[s].b_norm;
+      if (rn < 1e-30) {
+        sc[s].outer_status = first ? MTCOCR_X0_NULL : MTCOCR_OK;
+        sc[s].rel = (bn > 0.0) ? rn / bn : 0.0;
+      } else {
+        const double rel = rn / bn;
+        sc[s].rel = rel;
+        if (rel < eps) {
+          sc[s].outer_status = MTCOCR_OK;
+        } else if (!first && rel > 0.9 * sc[s].prev_rel) {
+          sc[s].outer_status = MTCOCR_ERR_NOTCONV;
+        } else if (sc[s].outer >= max_outer) {
+          sc[s].outer_status = MTCOCR_ERR_NOTCONV;
+        } else {
+          sc[s].prev_rel = rel;
+          sc[s].scale = 1.0 / rn;
+          // inner tolerance: what is still missing, with a safety factor, floored at the
+          // attainable FP32 level
+          double tau = 0.1 * eps / rel;
+          if (tau < inner_floor) tau = inner_floor;
+          if (tau > 0.1) tau = 0.1;
+          sc[s].eps = tau;
+          sc[s].outer += 1;
+        }
+      }
+    }
+    sc[s].status = (sc[s].outer_status == ST_RUNNING) ? ST_RUNNING : ST_INACTIVE;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int q = 0; q < SP; ++q) c += (sc[q].outer_status == ST_RUNNING);
+    *nrun_outer = c;
+    *nrun_inner = c;
+  }
+}
+
+// inner right-hand side b32 = r64 * scale, x32 = 0
+template <int SP>
+__global__ void k_to_inner(const double2* r64, float2* b32, float2* x32, const SysScalars* sc,
+                           int nb) {
+  const long long N = (long long)nb * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = int(t % SP);
+    if (sc[s].outer_status != ST_RUNNING) continue;
+    const double sc_ = sc[s].scale;
+    float2 o;
+    o.x = float(r64[t].x * sc_);
+    o.y = float(r64[t].y * sc_);
+    b32[t] = o;
+    x32[t] = make_float2(0.f, 0.f);
+  }
+}
+
+// x64 += y32 / scale (the MRS-smoothed inner iterate)
+template <int SP>
+__global__ void k_from_inner(double2* x64, const float2* y32, SysScalars* sc, int nb) {
+  const long long N = (long long)nb * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = int(t % SP);
+    if (sc[s].outer_status != ST_RUNNING) continue;
+    const double inv = 1.0 / sc[s].scale;
+    x64[t].x += double(y32[t].x) * inv;
+    x64[t].y += double(y32[t].y) * inv;
+  }
+}
+
+template <int SP>
+__global__ void s_inner_done(SysScalars* sc) {
+  const int s = threadIdx.x;
+  if (s < SP && sc[s].outer_status == ST_RUNNING) sc[s].total_iter += sc[s].iter;
+}
+
+// ------------------------------------------------------------------------------------------
+// device memory
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace mtcocr
+
+using namespace mtcocr;
+
+// ------------------------------------------------------------------------------------------
+// the handle
+
+struct mtcocr_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int* h_flags = nullptr;  // pinned: [0] running count
+
+  // matrix (shared index arrays, FP64 and FP32 values)
+  bool have_matrix = false;
+  bool ks_form = false;
+  int nb = 0, nslices = 0;
+  int64_t nnz_c = 0, nnz_r = 0;
+  DevBuf offc, nc, offr, nr, colc, colr;
+  DevBuf ac64, bc64, ar64, da64, db64;
+  DevBuf ac32, bc32, ar32;
+  // gradient
+  bool have_g = false;
+  int m = 0, wg = 0, wgt = 0;
+  DevBuf gcol, gtcol, gval64, gtval64, gval32, gtval32;
+
+  // batch state
+  int nsys = 0, sp = 0;
+  DevBuf omega, scal, nrun, nrun2, part;
+  DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
+  // COCR work vectors (type depends on the precision of the last solve)
+  DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vda, vdn, vx, vb;
+  int work_prec = -1, work_sp = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
+
+  int64_t device_bytes() const {
+    const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
+                           &db64, &ac32, &bc32, &ar32, &gcol, &gtcol, &gval64, &gtval64,
+                           &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &b64, &x64,
+                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vda, &vdn,
+                           &vx, &vb};
+    int64_t t = 0;
+    for (auto* b : all) t += (int64_t)b->bytes;
+    return t;
+  }
+};
+
+namespace {
+
+thread_local char g_err[512];
+
+#define HIPC(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      std::snprintf(g_err, sizeof(g_err), "%s failed: %s", #expr, hipGetErrorString(e_)); \
+      return e_ == hipErrorOutOfMemory ? MTCOCR_ERR_ALLOC : MTCOCR_ERR_HIP;               \
+    }                                                                                     \
+  } while (0)
+
+void drop_graph(mtcocr_handle* h) {
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  h->graph = nullptr;
+}
+
+template <class T>
+int upload(mtcocr_handle* h, DevBuf& d, const std::vector<T>& v, hipStream_t st) {
+  drop_graph(h);
+  HIPC(d.alloc(v.size() * sizeof(T)));
+  if (!v.empty()) HIPC(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+int upload_cast_float(mtcocr_handle* h, DevBuf& d, const std::vector<double>& v, hipStream_t st) {
+  drop_graph(h);
+  std::vector<float> f(v.size());
+  for (size_t i = 0; i < v.size(); ++i) f[i] = float(v[i]);
+  HIPC(d.alloc(f.size() * sizeof(float)));
+  if (!f.empty()) HIPC(hipMemcpyAsync(d.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  return 0;
+}
+
+int pad_sp(int nsys) {
+  int sp = 1;
+  while (sp < nsys) sp <<= 1;
+  return sp;
+}
+
+template <class T>
+EllView<T> ell_view(mtcocr_handle* h) {
+  EllView<T> v;
+  v.offc = h->offc.as<int>();
+  v.nc = h->nc.as<int>();
+  v.offr = h->offr.as<int>();
+  v.nr = h->nr.as<int>();
+  v.colc = h->colc.as<int>();
+  v.colr = h->colr.as<int>();
+  if constexpr (std::is_same<T, double>::value) {
+    v.ac = h->ac64.as<double>();
+    v.bc = h->bc64.as<double>();
+    v.ar = h->ar64.as<double>();
+  } else {
+    v.ac = h->ac32.as<float>();
+    v.bc = h->bc32.as<float>();
+    v.ar = h->ar32.as<float>();
+  }
+  v.nb = h->nb;
+  return v;
+}
+
+template <class T>
+GView<T> g_view(mtcocr_handle* h) {
+  GView<T> g;
+  g.gcol = h->gcol.as<int>();
+  g.gtcol = h->gtcol.as<int>();
+  if constexpr (std::is_same<T, double>::value) {
+    g.gval = h->gval64.as<double>();
+    g.gtval = h->gtval64.as<double>();
+  } else {
+    g.gval = h->gval32.as<float>();
+    g.gtval = h->gtval32.as<float>();
+  }
+  g.m = h->m;
+  g.n = h->nb;
+  g.wg = h->wg;
+  g.wgt = h->wgt;
+  return g;
+}
+
+// COCR engine over SP systems in storage precision T.
+template <class T, int SP>
+struct Engine {
+  using T2 = typename Vec2<T>::type;
+  mtcocr_handle* h;
+  EllView<T> A;
+  GView<T> G;
+  T2 *x, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *da, *dn;
+  SysScalars* sc;
+  int* nrun;
+  double* part;
+  const double* omega;
+  int nb, m, grid_e, grid_n;
+
+  explicit Engine(mtcocr_handle* hh) : h(hh) {
+    A = ell_view<T>(h);
+    G = g_view<T>(h);
+    nb = h->nb;
+    m = h->m;
+    x = h->vx.as<T2>();
+    b = h->vb.as<T2>();
+    r = h->vr.as<T2>();
+    p = h->vp.as<T2>();
+    s = h->vs.as<T2>();
+    z = h->vz.as<T2>();
+    a = h->va.as<T2>();
+    w = h->vw.as<T2>();
+    ym = h->vym.as<T2>();
+    sm = h->vsm.as<T2>();
+    help = h->vhelp.as<T2>();
+    da = h->vda.as<T2>();
+    dn = h->vdn.as<T2>();
+    sc = h->scal.as<SysScalars>();
+    nrun = h->nrun.as<int>();
+    part = h->part.as<double>();
+    omega = h->omega.as<double>();
+    grid_e = grid_for((long long)nb * SP);
+    grid_n = grid_for((long long)m * SP);
+  }
+
+  void setup_diagonals(hipStream_t st) {
+    const int g1 = std::min(4096, (int)(((long long)nb * SP + 255) / 256));
+    k_diag_a<T, SP><<<g1, 256, 0, st>>>(h->da64.as<double>(), h->db64.as<double>(), omega, da, nb);
+    const int g2 = std::min(4096, (int)(((long long)m * SP + 255) / 256) + 1);
+    k_diag_n<T, SP><<<g2, 256, 0, st>>>(ell_view<double>(h), g_view<double>(h), omega, dn);
+  }
+
+  // stages 1-4 of the request order: r = b - A x, X0 test, s = M^-1 r, p = a = w = 0, z = A s
+  void prologue(hipStream_t st, int maxiter) {
+    k_spmv<T, SP, 1><<<grid_e, kBlock, 0, st>>>(A, omega, x, r, b, ym, sm, sc, nrun, part);
+    s_prologue<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    k_pc_node<T, SP, false><<<grid_n, kBlock, 0, st>>>(G, r, nullptr, dn, help, sc, nrun);
+    k_pc_edge_plain<T, SP><<<grid_e, kBlock, 0, st>>>(G, r, da, help, s, sc, nrun);
+    const long long nv = (long long)nb * SP;
+    const int gf = std::min<long long>(2048, (nv + 255) / 256);
+    k_fill<T2><<<gf, 256, 0, st>>>(p, nv);
+    k_fill<T2><<<gf, 256, 0, st>>>(a, nv);
+    k_fill<T2><<<gf, 256, 0, st>>>(w, nv);
+    k_spmv<T, SP, 0><<<grid_e, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
+                                                nrun, part);
+    s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun, maxiter);
+  }
+
+  // one COCR iteration (stages 5-8)
+  void iteration(hipStream_t st, int maxiter) {
+    k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
+    k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, s, p, w, da, help, sc, nrun, part);
+    s_alpha<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    k_xr_mrs1<T, SP><<<grid_e, kBlock, 0, st>>>(x, p, r, a, sm, sc, nrun, part, nb);
+    s_mrs<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc);
+    k_mrs2<T, SP><<<grid_e, kBlock, 0, st>>>(x, r, ym, sm, s, w, sc, nrun, part, nb);
+    s_stop<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    k_spmv<T, SP, 0><<<grid_e, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
+                                                nrun, part);
+    s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun, maxiter);
+  }
+};
+
+int ensure(mtcocr_handle* h, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return 0;
+  drop_graph(h);  // captured kernels hold the old pointers
+  HIPC(b.alloc(bytes));
+  return 0;
+}
+
+int alloc_work(mtcocr_handle* h, int prec, int sp) {
+  const size_t es = (prec == MTCOCR_PREC_FP64) ? sizeof(double2) : sizeof(float2);
+  const size_t nv = (size_t)h->nb * sp * es;
+  const size_t nm = (size_t)std::max(h->m, 1) * sp * es;
+  int rc = 0;
+  DevBuf* vecs[] = {&h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw, &h->vym, &h->vsm, &h->vda,
+                    &h->vx, &h->vb};
+  for (DevBuf* b : vecs)
+    if ((rc = ensure(h, *b, nv))) return rc;
+  if ((rc = ensure(h, h->vhelp, nm))) return rc;
+  if ((rc = ensure(h, h->vdn, nm))) return rc;
+  h->work_prec = prec;
+  h->work_sp = sp;
+  return 0;
+}
+
+bool use_graphs() {
+  const char* e = std::getenv("MTCOCR_NO_GRAPH");
+  return !(e && e[0] == '1');
+}
+
+// Run COCR iterations until every system of the engine has left the RUNNING state.
+template <class T, int SP>
+int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter) {
+  hipStream_t st = h->stream;
+  const int prec = std::is_same<T, double>::value ? 0 : 1;
+  const int chunk = 8;
+  if (use_graphs()) {
+    if (!(h->graph && h->graph_prec == prec && h->graph_sp == SP && h->graph_chunk == chunk &&
+          h->graph_maxiter == maxiter)) {
+      if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+      }
+      hipGraph_t g;
+      HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter);
+      HIPC(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(e);
+      h->graph_prec = prec;
+      h->graph_sp = SP;
+      h->graph_chunk = chunk;
+      h->graph_maxiter = maxiter;
+    }
+  }
+  // every running system leaves RUNNING after at most maxiter + 1 iterations; the bound below
+  // only guards against a device that stopped executing the kernels
+  const long long cap = (long long)maxiter + 2LL * chunk + 16;
+  long long launched = 0;
+  for (;;) {
+    HIPC(hipMemcpyAsync(h->h_flags, E.nrun, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (h->h_flags[0] == 0) break;
+    if (launched > cap) {
+      std::snprintf(g_err, sizeof(g_err), "solver did not retire its systems after %lld iterations",
+                    launched);
+      return MTCOCR_ERR_HIP;
+    }
+    if (use_graphs()) {
+      HIPC(hipGraphLaunch(h->graph, st));
+    } else {
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter);
+    }
+    HIPC(hipGetLastError());
+    launched += chunk;
+  }
+  return 0;
+}
+
+void set_eps_status(mtcocr_handle* h, std::vector<SysScalars>& hs, int nsys, int sp, double eps) {
+  hs.assign(sp, SysScalars());
+  for (int s = 0; s < sp; ++s) {
+    std::memset(&hs[s], 0, sizeof(SysScalars));
+    hs[s].status = (s < nsys) ? ST_RUNNING : ST_INACTIVE;
+    hs[s].outer_status = hs[s].status;
+    hs[s].eps = eps;
+    hs[s].rel = 1.0;
+  }
+  (void)h;
+}
+
+template <int SP>
+int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
+  hipStream_t st = h->stream;
+  int rc = alloc_work(h, MTCOCR_PREC_FP64, SP);
+  if (rc) return rc;
+  Engine<double, SP> E(h);
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, SP, eps);
+  HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(E.x, h->x64.p, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  HIPC(hipMemcpyAsync(E.b, h->b64.p, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  // timed span: DFP setup (diagonals) + COCR loop, like the reference's kit interval
+  HIPC(hipEventRecord(h->ev0, st));
+  E.setup_diagonals(st);
+  E.prologue(st, maxiter);
+  HIPC(hipGetLastError());
+  rc = run_iterations<double, SP>(h, E, maxiter);
+  if (rc) return rc;
+  HIPC(hipEventRecord(h->ev1, st));
+  HIPC(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  // solution back into x64, true residual in FP64
+  HIPC(hipMemcpyAsync(h->x64.p, E.x, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  // true residual with all systems active
+  std::vector<SysScalars> tmp(hs);
+  for (int s = 0; s < SP; ++s) tmp[s].status = (s < h->nsys) ? ST_RUNNING : ST_INACTIVE;
+  HIPC(hipMemcpyAsync(E.sc, tmp.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  k_spmv<double, SP, 1><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.x, h->r64.as<double2>(), E.b,
+                                                     nullptr, nullptr, E.sc, E.nrun, E.part);
+  HIPC(hipGetLastError());
+  std::vector<double> partials((size_t)E.grid_e * SP * 2);
+  HIPC(hipMemcpyAsync(partials.data(), E.part, partials.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  for (int s = 0; s < h->nsys; ++s) {
+    double bb = 0, rr = 0;
+    for (int b = 0; b < E.grid_e; ++b) {
+      bb += partials[((size_t)b * SP + s) * 2];
+      rr += partials[((size_t)b * SP + s) * 2 + 1];
+    }
+    mtcocr_stats& o = stats[s];
+    o.req = hs[s].status == ST_RUNNING ? MTCOCR_MAXITER : hs[s].status;
+    o.iter = hs[s].iter;
+    o.outer = 1;
+    o.residual_rel = hs[s].rel;
+    o.true_rel_res = bb > 0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
+    o.seconds = ms * 1e-3;
+  }
+  return 0;
+}
+
+template <int SP>
+int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
+  hipStream_t st = h->stream;
+  int rc = alloc_work(h, MTCOCR_PREC_MIXED, SP);
+  if (rc) return rc;
+  Engine<float, SP> E(h);
+  EllView<double> A64 = ell_view<double>(h);
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, SP, eps);
+  HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
+  int one = 1;
+  int* nrun_outer = h->nrun2.as<int>();
+  HIPC(hipMemcpyAsync(nrun_outer, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  const double inner_floor = 1e-5;
+  const int max_outer = 40;
+  HIPC(hipEventRecord(h->ev0, st));
+  E.setup_diagonals(st);
+  const long long nv = (long long)h->nb * SP;
+  const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
+  for (int pass = 0;; ++pass) {
+    if (pass > max_outer + 1) {
+      std::snprintf(g_err, sizeof(g_err), "refinement did not finish after %d passes", pass);
+      return MTCOCR_ERR_HIP;
+    }
+    // FP64 true residual r64 = b - A x64 on the systems still refining
+    k_spmv<double, SP, 2><<<E.grid_e, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
+                                                       h->r64.as<double2>(), h->b64.as<double2>(),
+                                                       nullptr, nullptr, E.sc, nrun_outer, E.part);
+    s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e, E.sc, nrun_outer, E.nrun, eps,
+                                              inner_floor, pass == 0 ? 1 : 0, max_outer);
+    HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (h->h_flags[0] == 0) break;
+    k_to_inner<SP><<<gv, 256, 0, st>>>(h->r64.as<double2>(), E.b, E.x, E.sc, h->nb);
+    E.prologue(st, maxiter);
+    HIPC(hipGetLastError());
+    rc = run_iterations<float, SP>(h, E, maxiter);
+    if (rc) return rc;
+    s_inner_done<SP><<<1, 64, 0, st>>>(E.sc);
+    k_from_inner<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), E.ym, E.sc, h->nb);
+  }
+  HIPC(hipEventRecord(h->ev1, st));
+  HIPC(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  for (int s = 0; s < h->nsys; ++s) {
+    mtcocr_stats& o = stats[s];
+    o.req = hs[s].outer_status == ST_RUNNING ? MTCOCR_ERR_NOTCONV : hs[s].outer_status;
+    o.iter = hs[s].total_iter;
+    o.outer = hs[s].outer;
+    o.residual_rel = hs[s].rel;
+    o.true_rel_res = hs[s].rel;
+    o.seconds = ms * 1e-3;
+  }
+  return 0;
+}
+
+template <class F>
+int with_sp(int sp, F&& f) {
+  switch (sp) {
+    case 1: return f(std::integral_constant<int, 1>());
+    case 2: return f(std::integral_constant<int, 2>());
+    case 4: return f(std::integral_constant<int, 4>());
+    case 8: return f(std::integral_constant<int, 8>());
+    case 16: return f(std::integral_constant<int, 16>());
+    case 32: return f(std::integral_constant<int, 32>());
+    case 64: return f(std::integral_constant<int, 64>());
+  }
+  return MTCOCR_ERR_ARG;
+}
+
+int set_batch(mtcocr_handle* h, int nsys, const double* omega) {
+  if (nsys < 1 || nsys > 64) return MTCOCR_ERR_ARG;
+  const int sp = pad_sp(nsys);
+  std::vector<double> om(sp, 0.0);
+  for (int s = 0; s < nsys; ++s) om[s] = omega ? omega[s] : 1.0;
+  int rc;
+  if ((rc = ensure(h, h->omega, sizeof(double) * 64))) return rc;
+  if ((rc = ensure(h, h->scal, sizeof(SysScalars) * 64))) return rc;
+  if ((rc = ensure(h, h->nrun, 64))) return rc;
+  if ((rc = ensure(h, h->nrun2, 64))) return rc;
+  if ((rc = ensure(h, h->part, sizeof(double) * (size_t)kMaxBlocks * 64 * 2))) return rc;
+  const size_t v64 = (size_t)h->nb * sp * sizeof(double2);
+  if ((rc = ensure(h, h->b64, v64))) return rc;
+  if ((rc = ensure(h, h->x64, v64))) return rc;
+  if ((rc = ensure(h, h->r64, v64))) return rc;
+  HIPC(hipMemcpyAsync(h->omega.p, om.data(), sizeof(double) * sp, hipMemcpyHostToDevice, h->stream));
+  h->nsys = nsys;
+  h->sp = sp;
+  return 0;
+}
+
+// interleave nsys host vectors (each 2*nb doubles) into the [row][system] device layout
+int put_batch(mtcocr_handle* h, DevBuf& dst, const double* src) {
+  const int sp = h->sp, nb = h->nb;
+  std::vector<double2> tmp((size_t)nb * sp);
+  for (int r = 0; r < nb; ++r)
+    for (int s = 0; s < sp; ++s) {
+      double2 v;
+      if (s < h->nsys && src) {
+        v.x = src[(size_t)s * 2 * nb + 2 * r];
+        v.y = src[(size_t)s * 2 * nb + 2 * r + 1];
+      } else {
+        v.x = 0.0;
+        v.y = 0.0;
+      }
+      tmp[(size_t)r * sp + s] = v;
+    }
+  HIPC(hipMemcpyAsync(dst.p, tmp.data(), tmp.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int get_batch(mtcocr_handle* h, const DevBuf& srcb, double* dst) {
+  const int sp = h->sp, nb = h->nb;
+  std::vector<double2> tmp((size_t)nb * sp);
+  HIPC(hipMemcpyAsync(tmp.data(), srcb.p, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  for (int r = 0; r < nb; ++r)
+    for (int s = 0; s < h->nsys; ++s) {
+      dst[(size_t)s * 2 * nb + 2 * r] = tmp[(size_t)r * sp + s].x;
+      dst[(size_t)s * 2 * nb + 2 * r + 1] = tmp[(size_t)r * sp + s].y;
+    }
+  return 0;
+}
+
+int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
+  hipStream_t st = h->stream;
+  int rc;
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  h->have_matrix = false;
+  h->nb = E.nb;
+  h->nslices = E.nslices;
+  h->nnz_c = E.nnz_complex;
+  h->nnz_r = E.nnz_real;
+  h->ks_form = ks;
+  if ((rc = upload(h, h->offc, E.offc, st)) || (rc = upload(h, h->nc, E.nc, st)) ||
+      (rc = upload(h, h->offr, E.offr, st)) || (rc = upload(h, h->nr, E.nr, st)) ||
+      (rc = upload(h, h->colc, E.colc, st)) || (rc = upload(h, h->colr, E.colr, st)) ||
+      (rc = upload(h, h->ac64, E.ac, st)) || (rc = upload(h, h->bc64, E.bc, st)) ||
+      (rc = upload(h, h->ar64, E.ar, st)) || (rc = upload(h, h->da64, E.diag_a, st)) ||
+      (rc = upload(h, h->db64, E.diag_b, st)))
+    return rc;
+  HIPC(hipStreamSynchronize(st));
+  if ((rc = upload_cast_float(h, h->ac32, E.ac, st)) ||
+      (rc = upload_cast_float(h, h->bc32, E.bc, st)) ||
+      (rc = upload_cast_float(h, h->ar32, E.ar, st)))
+    return rc;
+  h->have_matrix = true;
+  h->nsys = 0;
+  h->work_prec = -1;
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C-ABI
+
+extern "C" {
+
+const char* mtcocr_error_string(int code) {
+  switch (code) {
+    case MTCOCR_OK: return "converged";
+    case MTCOCR_X0_NULL: return "initial guess is already the solution";
+    case MTCOCR_DIV_BY_ZERO: return "division by zero inside COCR";
+    case MTCOCR_MAXITER: return "maximum number of iterations reached";
+    case MTCOCR_ERR_ARG: return "invalid argument";
+    case MTCOCR_ERR_HIP: return g_err[0] ? g_err : "HIP runtime error";
+    case MTCOCR_ERR_ALLOC: return "device memory allocation failed";
+    case MTCOCR_ERR_STATE: return "matrix, gradient or right-hand side not set";
+    case MTCOCR_ERR_NOTCONV: return "iterative refinement stalled above the tolerance";
+  }
+  return "unknown error";
+}
+
+int mtcocr_create(int device, mtcocr_handle** out) {
+  if (!out) return MTCOCR_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  HIPC(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(device));
+  std::unique_ptr<mtcocr_handle> h(new mtcocr_handle());
+  h->device = device;
+  HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPC(hipEventCreate(&h->ev0));
+  HIPC(hipEventCreate(&h->ev1));
+  HIPC(hipHostMalloc((void**)&h->h_flags, 64, hipHostMallocDefault));
+  *out = h.release();
+  return 0;
+}
+
+int mtcocr_destroy(mtcocr_handle* h) {
+  if (!h) return MTCOCR_ERR_ARG;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->h_flags) (void)hipHostFree(h->h_flags);
+  hipStream_t st = h->stream;
+  delete h;  // DevBuf destructors free device memory
+  if (st) (void)hipStreamDestroy(st);
+  return 0;
+}
+
+int mtcocr_set_matrix_block2x2(mtcocr_handle* h, int nb, const int* ig, const int* jg,
+                               const int* idi, const int* ijg, const double* di,
+                               const double* gg) {
+  if (!h || nb <= 0) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  SlicedEll E;
+  int rc = ell_from_block2x2(nb, ig, jg, idi, ijg, di, gg, E);
+  if (rc) return rc;
+  return upload_matrix(h, E, false);
+}
+
+int mtcocr_set_matrix_ks(mtcocr_handle* h, int nb, const int* ig, const int* jg,
+                         const double* k_di, const double* k_gg, const double* s_di,
+                         const double* s_gg) {
+  if (!h || nb <= 0) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  SlicedEll E;
+  int rc = ell_from_ks(nb, ig, jg, k_di, k_gg, s_di, s_gg, E);
+  if (rc) return rc;
+  return upload_matrix(h, E, true);
+}
+
+int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_nodes_c,
+                             const int* nvkat, const int* nver, const double* sigma3d,
+                             const int* edges, const int* ig_t, const int* jg_t,
+                             const double* gg_t, const int* is_node_bound) {
+  if (!h) return MTCOCR_ERR_ARG;
+  if (h->have_matrix && n_edges != h->nb) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  GradientEll G;
+  int rc = build_gradient(kpar, n_edges, n_nodes_c, nvkat, nver, sigma3d, edges, ig_t, jg_t, gg_t,
+                          is_node_bound, G);
+  if (rc) return rc;
+  if (G.wg > 8) return MTCOCR_ERR_ARG;  // more than 8 edges at a node: not a hexahedral mesh
+  hipStream_t st = h->stream;
+  h->have_g = false;
+  if ((rc = upload(h, h->gcol, G.gcol, st)) || (rc = upload(h, h->gtcol, G.gtcol, st)) ||
+      (rc = upload(h, h->gval64, G.gval, st)) || (rc = upload(h, h->gtval64, G.gtval, st)))
+    return rc;
+  HIPC(hipStreamSynchronize(st));
+  if ((rc = upload_cast_float(h, h->gval32, G.gval, st)) ||
+      (rc = upload_cast_float(h, h->gtval32, G.gtval, st)))
+    return rc;
+  h->m = G.m;
+  h->wg = G.wg;
+  h->wgt = G.wgt;
+  h->have_g = true;
+  h->work_prec = -1;
+  return 0;
+}
+
+int mtcocr_set_rhs(mtcocr_handle* h, int nsys, const double* omega, const double* pr,
+                   const double* x0) {
+  if (!h || !pr) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix) return MTCOCR_ERR_STATE;
+  if (!h->ks_form && (nsys != 1 || omega)) return MTCOCR_ERR_ARG;
+  if (h->ks_form && !omega) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  int rc = set_batch(h, nsys, omega);
+  if (rc) return rc;
+  if ((rc = put_batch(h, h->b64, pr))) return rc;
+  if ((rc = put_batch(h, h->x64, x0))) return rc;
+  return 0;
+}
+
+int mtcocr_solve_resident(mtcocr_handle* h, double eps, int maxiter, int precision,
+                          mtcocr_stats* stats) {
+  if (!h || !stats || maxiter < 0 || !(eps > 0)) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  int rc = with_sp(h->sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    if (precision == MTCOCR_PREC_FP64) return solve_fp64<SP>(h, eps, maxiter, stats);
+    return solve_mixed<SP>(h, eps, maxiter, stats);
+  });
+  if (rc) return rc;
+  for (int s = 0; s < h->nsys; ++s)
+    if (stats[s].req != 0) return stats[s].req;
+  return 0;
+}
+
+int mtcocr_get_solution(mtcocr_handle* h, double* x) {
+  if (!h || !x) return MTCOCR_ERR_ARG;
+  if (h->nsys < 1) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  return get_batch(h, h->x64, x);
+}
+
+int mtcocr_solve_batch(mtcocr_handle* h, int nsys, const double* omega, const double* pr,
+                       double* x, double eps, int maxiter, int precision, mtcocr_stats* stats) {
+  if (!h || !pr || !x || !stats) return MTCOCR_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = mtcocr_set_rhs(h, nsys, omega, pr, x);
+  if (rc) return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  const int res = mtcocr_solve_resident(h, eps, maxiter, precision, stats);
+  if (res < 0 && res <= MTCOCR_ERR_ARG && res != MTCOCR_ERR_NOTCONV) return res;
+  auto t2 = std::chrono::steady_clock::now();
+  rc = mtcocr_get_solution(h, x);
+  if (rc) return rc;
+  auto t3 = std::chrono::steady_clock::now();
+  const double io = std::chrono::duration<double>(t1 - t0).count() +
+                    std::chrono::duration<double>(t3 - t2).count();
+  for (int s = 0; s < nsys; ++s) stats[s].seconds_io = io;
+  return res;
+}
+
+int mtcocr_solve(mtcocr_handle* h, const double* pr, double* x, double eps, int maxiter,
+                 int precision, mtcocr_stats* stats) {
+  if (!h) return MTCOCR_ERR_ARG;
+  if (h->ks_form) return MTCOCR_ERR_STATE;
+  mtcocr_stats st;
+  std::memset(&st, 0, sizeof(st));
+  const int rc = mtcocr_solve_batch(h, 1, nullptr, pr, x, eps, maxiter, precision, &st);
+  if (stats) *stats = st;
+  return rc;
+}
+
+int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi, const int* ijg,
+                            const double* di, const double* gg, const double* pr, double* x,
+                            double eps, int maxiter, int kpar, int n_edges_c, int n_nodes_c,
+                            const double* xyz, const int* nvkat, const int* nver,
+                            const double* sigma3d, const int* edges, const int* ig_t,
+                            const int* jg_t, const double* gg_t, const int* is_node_bound,
+                            mtcocr_stats* stats) {
+  (void)xyz;  // unused by the reference as well (only passed through)
+  if (n <= 0 || n % 2 || n / 2 != n_edges_c) return MTCOCR_ERR_ARG;
+  int dev = 0;
+  if (const char* e = std::getenv("MTCOCR_DEVICE")) dev = std::atoi(e);
+  mtcocr_handle* h = nullptr;
+  int rc = mtcocr_create(dev, &h);
+  if (rc) return rc;
+  rc = mtcocr_set_matrix_block2x2(h, n / 2, ig, jg, idi, ijg, di, gg);
+  if (!rc)
+    rc = mtcocr_set_gradient_mesh(h, kpar, n_edges_c, n_nodes_c, nvkat, nver, sigma3d, edges,
+                                  ig_t, jg_t, gg_t, is_node_bound);
+  if (!rc) rc = mtcocr_solve(h, pr, x, eps, maxiter, MTCOCR_PREC_FP64, stats);
+  mtcocr_destroy(h);
+  return rc;
+}
+
+int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, double* avg_ms,
+                      double* bytes_per_launch) {
+  if (!h || reps < 1 || !avg_ms || !bytes_per_launch) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || h->nsys < 1 || nsys != h->nsys) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  const int sp = h->sp;
+  int rc = alloc_work(h, precision, sp);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, sp, 1e-8);
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  const long long nv = (long long)h->nb * sp;
+  const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
+  float ms = 0.f;
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    if (precision == MTCOCR_PREC_FP64) {
+      Engine<double, SP> E(h);
+      k_fill<double2><<<gf, 256, 0, st>>>(E.s, nv);
+      k_spmv<double, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
+                                                         nullptr, E.sc, E.nrun, E.part);
+      HIPC(hipEventRecord(h->ev0, st));
+      for (int i = 0; i < reps; ++i)
+        k_spmv<double, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
+                                                           nullptr, nullptr, E.sc, E.nrun, E.part);
+      HIPC(hipEventRecord(h->ev1, st));
+    } else {
+      Engine<float, SP> E(h);
+      k_fill<float2><<<gf, 256, 0, st>>>(E.s, nv);
+      k_spmv<float, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
+                                                        nullptr, E.sc, E.nrun, E.part);
+      HIPC(hipEventRecord(h->ev0, st));
+      for (int i = 0; i < reps; ++i)
+        k_spmv<float, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
+                                                          nullptr, nullptr, E.sc, E.nrun, E.part);
+      HIPC(hipEventRecord(h->ev1, st));
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipEventSynchronize(h->ev1));
+    HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    return 0;
+  });
+  if (rc) return rc;
+  *avg_ms = double(ms) / reps;
+  // algorithmic bytes: matrix values + column indices once, x read once and y written once per
+  // system, the slice descriptors, and the fused dot's re-read of x (served by the same lines)
+  const double es = (precision == MTCOCR_PREC_FP64) ? 8.0 : 4.0;
+  const double nnzc = double(h->nnz_c), nnzr = double(h->nnz_r);
+  const double val = (h->ks_form ? 2.0 * es * nnzc : 2.0 * es * nnzc) + es * nnzr;
+  const double idx = 4.0 * (nnzc + nnzr);
+  const double vec = double(nsys) * 2.0 * (2.0 * es * h->nb);
+  const double desc = 16.0 * h->nslices;
+  *bytes_per_launch = val + idx + vec + desc;
+  return 0;
+}
+
+int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, double* y) {
+  if (!h || !x || !y || (op != 0 && op != 1)) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || (op == 1 && !h->have_g)) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  int rc;
+  if (h->nsys < 1) {
+    if (h->ks_form) return MTCOCR_ERR_STATE;
+    if ((rc = set_batch(h, 1, nullptr))) return rc;
+  }
+  const int sp = h->sp;
+  if ((rc = alloc_work(h, precision, sp))) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, 1, sp, 1e-8);  // system 0 only
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  // x -> r64 (system 0 of the [row][system] layout), converted on the host for FP32
+  const int nb = h->nb;
+  std::vector<double2> in((size_t)nb * sp);
+  for (int r = 0; r < nb; ++r)
+    for (int s = 0; s < sp; ++s) in[(size_t)r * sp + s] = s == 0 ? make_double2(x[2 * r], x[2 * r + 1]) : make_double2(0, 0);
+  std::vector<double2> out((size_t)nb * sp);
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    auto run = [&](auto tag) -> int {
+      using T = decltype(tag);
+      using T2 = typename Vec2<T>::type;
+      Engine<T, SP> E(h);
+      std::vector<T2> xin((size_t)nb * SP), yout((size_t)nb * SP);
+      for (size_t k = 0; k < xin.size(); ++k) {
+        xin[k].x = T(in[k].x);
+        xin[k].y = T(in[k].y);
+      }
+      HIPC(hipMemcpyAsync(E.r, xin.data(), xin.size() * sizeof(T2), hipMemcpyHostToDevice, st));
+      if (op == 0) {
+        k_spmv<T, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.r, E.z, nullptr, nullptr,
+                                                      nullptr, E.sc, E.nrun, E.part);
+      } else {
+        E.setup_diagonals(st);
+        k_pc_node<T, SP, false><<<E.grid_n, kBlock, 0, st>>>(E.G, E.r, nullptr, E.dn, E.help,
+                                                             E.sc, E.nrun);
+        k_pc_edge_plain<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.r, E.da, E.help, E.z, E.sc,
+                                                            E.nrun);
+      }
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(yout.data(), E.z, yout.size() * sizeof(T2), hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      for (size_t k = 0; k < yout.size(); ++k) out[k] = make_double2(double(yout[k].x), double(yout[k].y));
+      return 0;
+    };
+    if (precision == MTCOCR_PREC_FP64) return run(double());
+    return run(float());
+  });
+  if (rc) return rc;
+  for (int r = 0; r < nb; ++r) {
+    y[2 * r] = out[(size_t)r * sp].x;
+    y[2 * r + 1] = out[(size_t)r * sp].y;
+  }
+  return 0;
+}
+
+int mtcocr_matrix_info(mtcocr_handle* h, int64_t* nnz_complex, int64_t* nnz_real, int64_t* m,
+                       int64_t* device_bytes) {
+  if (!h) return MTCOCR_ERR_ARG;
+  if (nnz_complex) *nnz_complex = h->nnz_c;
+  if (nnz_real) *nnz_real = h->nnz_r;
+  if (m) *m = h->m;
+  if (device_bytes) *device_bytes = h->device_bytes();
+  return 0;
+}
+
+}  // extern "C"

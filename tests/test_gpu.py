@@ -1,0 +1,212 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle (FP64 CPU restatement).
+
+Tolerances:
+  SpMV / preconditioner, FP64 storage: 1e-13 / 1e-11 of max|y| (summation order only)
+  SpMV / preconditioner, FP32 storage: 2e-6 / 2e-5 of max|y|
+  full FP64 solve: iteration count within 2% (+2) of the oracle, MRS residual < eps, true
+    residual <= 3e-8 (the reference's own returned x reaches 1.0e-8..2.9e-8), x within 1e-5
+    relative of the oracle's x (rounding growth through COCR on a system with 1e-8 S/m air)
+  mixed precision: true FP64 residual < eps = 1e-8, x within 1e-6 of the FP64 GPU solution's
+    distance to the oracle, i.e. ||x_mixed - x_oracle|| <= 1e-5 ||x_oracle||
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import identity_system
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def S(c1_system, c1_mesh):
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("create solver, config-1 system (45000 edges)")
+    s = Solver(0)
+    s.set_matrix(c1_system)
+    s.set_gradient(c1_mesh)
+    yield s
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def oracle_c1(O, c1_system, c1_mesh):
+    x, st, tr = O.pcocr(c1_system, c1_mesh, eps=1e-8, nthreads=1, trace=True)
+    return x, st
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("mixed", 2e-6)])
+def test_spmv_parity(S, O, M, c1_system, prec, tol):
+    say(f"spmv {prec}")
+    x = np.random.default_rng(11).standard_normal(2 * c1_system.nb)
+    ref = O.spmv(c1_system, x)
+    got = S.apply("spmv", x, prec)
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
+def test_preconditioner_parity(S, O, M, c1_system, c1_mesh, prec, tol):
+    say(f"preconditioner {prec}")
+    d = O.Dfp(c1_mesh, c1_system)
+    x = np.random.default_rng(12).standard_normal(2 * c1_system.nb)
+    ref = d.apply(x)
+    got = S.apply("precond", x, prec)
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_fp64_solve_parity(S, O, M, c1_system, oracle_c1):
+    say("fp64 solve")
+    xo, so = oracle_c1
+    x, st = S.solve(c1_system.pr, eps=1e-8, maxiter=10000, precision="fp64")
+    say(f"gpu {st} oracle iter {so['iter']}")
+    assert st["req"] == 0 and so["req"] == 0
+    assert abs(st["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+    assert st["residual_rel"] < 1e-8
+    A = M.sparse_matrix(c1_system)
+    b = M.to_complex(c1_system.pr)
+    true_rel = np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b)
+    assert true_rel <= 3e-8
+    assert st["true_rel_res"] == pytest.approx(true_rel, rel=1e-3)
+    assert rel(x, xo) <= 1e-5
+
+
+def test_fp64_deterministic(S, c1_system):
+    say("determinism")
+    x1, s1 = S.solve(c1_system.pr, eps=1e-8, precision="fp64")
+    x2, s2 = S.solve(c1_system.pr, eps=1e-8, precision="fp64")
+    assert s1["iter"] == s2["iter"] and np.array_equal(x1, x2)
+
+
+def test_mixed_solve(S, O, M, c1_system, oracle_c1):
+    say("mixed solve")
+    xo, so = oracle_c1
+    x, st = S.solve(c1_system.pr, eps=1e-8, maxiter=10000, precision="mixed")
+    say(f"mixed {st}")
+    assert st["req"] == 0
+    A = M.sparse_matrix(c1_system)
+    b = M.to_complex(c1_system.pr)
+    true_rel = np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b)
+    assert true_rel < 1e-8
+    assert rel(x, xo) <= 1e-5
+
+
+def test_batched_ks_matches_single(O, M, c1_model, c1_mesh):
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("batched K/S: 2 frequencies x 2 polarizations")
+    freqs = [0.1, 10.0]
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    with Solver(0) as B:
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        xb, sb = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        xm, sm = B.solve_batch(om, prs, eps=1e-8, precision="mixed")
+    for k, (f, a) in enumerate(cases):
+        s = M.assemble(c1_model, f, a)
+        xo, so, _ = O.pcocr(s, c1_mesh, eps=1e-8)
+        say(f"case {f} Hz alpha={a}: batch {sb[k]['iter']} it, oracle {so['iter']} it, "
+            f"mixed {sm[k]['iter']} it / {sm[k]['outer']} passes")
+        assert sb[k]["req"] == 0 and sm[k]["req"] == 0
+        assert abs(sb[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(xb[k], xo) <= 1e-5
+        A = M.sparse_matrix(s)
+        b = M.to_complex(s.pr)
+        assert np.linalg.norm(b - A @ M.to_complex(xm[k])) / np.linalg.norm(b) < 1e-8
+
+
+def test_special_cases(S, M, c1_system, c1_mesh):
+    say("special cases")
+    x, st = S.solve(np.zeros_like(c1_system.pr), precision="fp64")
+    assert st["req"] == -1 and not x.any()
+    x, st = S.solve(c1_system.pr, maxiter=1, precision="fp64")
+    assert st["req"] == -3 and st["iter"] == 2
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    m, ma, s = identity_system(M)
+    with Solver(0) as T:
+        T.set_matrix(s)
+        T.set_gradient(ma)
+        x, st = T.solve(s.pr, precision="fp64")
+        assert st["req"] == -2 and st["iter"] == 1
+
+
+def test_pcocr_2x2_folded_mirror(S, M, c1_system, c1_mesh):
+    """The reference-signature entry point gives the same x as the handle API (deterministic)."""
+    say("pcocr_2x2_folded mirror")
+    from mtcalc_with_dfp_cocr_amd.cocr import pcocr_2x2_folded
+
+    s, ma = c1_system, c1_mesh
+    x = np.zeros(2 * s.nb)
+    info = {}
+    rc = pcocr_2x2_folded(2 * s.nb, s.ig, s.jg, s.idi, s.ijg, s.di, s.gg, s.pr, x, 1e-8, 10000,
+                          ma.kpar, s.nb, ma.kuzlov, ma.xyz, ma.nvkat, ma.nver, ma.sig_flags,
+                          ma.edges, is_node_bound=ma.is_node_bound, stats=info)
+    xh, st = S.solve(s.pr, eps=1e-8, precision="fp64")
+    assert rc == 0 and info["iter"] == st["iter"] and np.array_equal(x, xh)
+
+
+EXE = os.path.join(os.path.dirname(__file__), "..", "mtcalc_with_dfp_cocr_amd", "lib", "COCR_FP")
+
+
+def test_dropin_executable(O, M, c1_model, c1_system, c1_mesh, tmp_path):
+    """COCR_FP in a working directory: same files in, v3.dat / kit out, exit 0 on
+    non-convergence, exit 2 with --strict."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    say("drop-in executable")
+    d = M.write_cocr_fp_dir(tmp_path / "Nu1_X", c1_model, c1_system, mesh=c1_mesh)
+    # capitalised names as RegularMeshBuilder writes them
+    os.rename(d / "sig3d", d / "Sig3d")
+    os.rename(d / "l13d.dat", d / "L13d.dat")
+    os.rename(d / "mu3d", d / "mu3D")
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    v3 = np.fromfile(d / "v3.dat", np.float64)
+    kit = (d / "kit").read_text()
+    assert kit.startswith("r=") and "\teps=1.000000e-08\tkit=" in kit and kit.rstrip().endswith("sec")
+    xo, so, _ = O.pcocr(c1_system, c1_mesh, eps=1e-8)
+    assert len(v3) == 2 * c1_system.nb and rel(v3, xo) <= 1e-5
+    # non-convergence: exit 0 like the reference, 2 with --strict
+    (d / "kuslau").write_text(f"{2 * c1_system.nb}\n1.000000e-08\n3\n")
+    assert subprocess.run([EXE], cwd=d, capture_output=True, timeout=120).returncode == 0
+    assert subprocess.run([EXE, "--strict"], cwd=d, capture_output=True, timeout=120).returncode == 2
+    # missing input: error exit
+    os.remove(d / "gg")
+    assert subprocess.run([EXE], cwd=d, capture_output=True, timeout=120).returncode == 1
+
+
+def test_dropin_batch_executable(O, M, c1_model, c1_mesh, tmp_path):
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    say("drop-in executable --batch")
+    dirs = []
+    for f, a in [(0.1, 1), (0.1, 0), (10.0, 1)]:
+        s = M.assemble(c1_model, f, a)
+        dirs.append(M.write_cocr_fp_dir(tmp_path / f"Nu{f}_{a}", c1_model, s, mesh=c1_mesh))
+    r = subprocess.run([EXE, "--batch", *map(str, dirs)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    for d in dirs:
+        s = M.read_block_system(d)
+        xo, so, _ = O.pcocr(s, c1_mesh, eps=1e-8)
+        assert rel(np.fromfile(d / "v3.dat", np.float64), xo) <= 1e-5

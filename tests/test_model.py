@@ -1,0 +1,281 @@
+"""Model layer (mesh numbering, portrait, element matrices, 1-D normal field, assembly).
+
+Every check is independent of the implementation under test: brute-force enumeration, an
+independent quadrature of the edge basis, the curl-of-gradient null space, exact reproduction of
+a linear field, the analytic half-space solution of the 1-D problem, and the K + i*omega*S split.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+
+# ---- numbering / topology ----------------------------------------------------------------
+
+def _grid(nxn, nyn, nzn):
+    nodes = {}
+    for iz in range(nzn):
+        for iy in range(nyn):
+            for ix in range(nxn):
+                nodes[(ix, iy, iz)] = (iz * nyn + iy) * nxn + ix + 1
+    return nodes
+
+
+@pytest.mark.parametrize("shape", [(3, 4, 5), (2, 2, 2), (5, 2, 3)])
+def test_edges_are_lexicographic_node_pairs(M, shape):
+    """Edge ids = rank of (node1, node2) in lexicographic order (RegularMeshBuilder set)."""
+    import ctypes as C
+
+    from mtcalc_with_dfp_cocr_amd._lib import lib, ptr
+
+    nxn, nyn, nzn = shape
+    nodes = _grid(*shape)
+    pairs = set()
+    for (ix, iy, iz), n in nodes.items():
+        for d in [(1, 0, 0), (0, 1, 0), (0, 0, 1)]:
+            q = (ix + d[0], iy + d[1], iz + d[2])
+            if q in nodes:
+                pairs.add((n, nodes[q]))
+    expect = np.array(sorted(pairs), dtype=np.int32).ravel()
+    nb = lib().mtmodel_num_edges(nxn, nyn, nzn)
+    got = np.empty(2 * nb, np.int32)
+    kt1 = C.c_int64()
+    assert lib().mtmodel_topology(nxn, nyn, nzn, None, None, ptr(got), None, C.byref(kt1)) == 0
+    np.testing.assert_array_equal(got, expect)
+    # boundary node count of the reference's KT1 formula (RegularMeshBuilder/main.cpp:1311)
+    assert kt1.value == (nxn * nyn + nxn * (nzn - 2) + (nyn - 2) * (nzn - 2)) * 2
+
+
+def test_cell_edges_follow_local_order(M):
+    """edges.dat rows: 12 edges in the local order x(y-,z-),x(y+,z-),x(y-,z+),x(y+,z+),
+    y(x-,z-),y(x-,z+),y(x+,z-),y(x+,z+), z(x-,y-),z(x+,y-),z(x-,y+),z(x+,y+)."""
+    import ctypes as C
+
+    from mtcalc_with_dfp_cocr_amd._lib import lib, ptr
+
+    shape = (4, 3, 3)
+    nxn, nyn, nzn = shape
+    nodes = _grid(*shape)
+    pairs = sorted({(nodes[p], nodes[q]) for p in nodes for q in nodes
+                    if nodes[q] > nodes[p] and sum(abs(a - b) for a, b in zip(p, q)) == 1})
+    eid = {pq: k + 1 for k, pq in enumerate(pairs)}
+    kpar = (nxn - 1) * (nyn - 1) * (nzn - 1)
+    ed = np.empty(kpar * 25, np.int32)
+    nver = np.empty(kpar * 14, np.int32)
+    kt1 = C.c_int64()
+    assert lib().mtmodel_topology(nxn, nyn, nzn, ptr(nver), ptr(ed), None, None, C.byref(kt1)) == 0
+    ed = ed.reshape(kpar, 25)
+    nver = nver.reshape(kpar, 14)
+    cell = 0
+    for cz in range(nzn - 1):
+        for cy in range(nyn - 1):
+            for cx in range(nxn - 1):
+                n = lambda dx, dy, dz: nodes[(cx + dx, cy + dy, cz + dz)]
+                want = [(n(0, 0, 0), n(1, 0, 0)), (n(0, 1, 0), n(1, 1, 0)), (n(0, 0, 1), n(1, 0, 1)),
+                        (n(0, 1, 1), n(1, 1, 1)), (n(0, 0, 0), n(0, 1, 0)), (n(0, 0, 1), n(0, 1, 1)),
+                        (n(1, 0, 0), n(1, 1, 0)), (n(1, 0, 1), n(1, 1, 1)), (n(0, 0, 0), n(0, 0, 1)),
+                        (n(1, 0, 0), n(1, 0, 1)), (n(0, 1, 0), n(0, 1, 1)), (n(1, 1, 0), n(1, 1, 1))]
+                assert list(ed[cell, :12]) == [eid[p] for p in want]
+                assert list(ed[cell, 12:]) == [0] * 12 + [1]
+                corners = [n(0, 0, 0), n(1, 0, 0), n(0, 1, 0), n(1, 1, 0), n(0, 0, 1), n(1, 0, 1),
+                           n(0, 1, 1), n(1, 1, 1)]
+                assert list(nver[cell, :8]) == corners and list(nver[cell, 8:]) == [0] * 6
+                cell += 1
+
+
+def test_portrait_matches_element_pairs(M, tiny_model):
+    """Lower portrait = union over cells of (edge, smaller edge) pairs; block size 2 exactly
+    for parallel pairs (T_Portrait::Gen_Portrait + Gen_idi_ijg)."""
+    ma = M.mesh_arrays(tiny_model)
+    p = M.portrait(tiny_model)
+    ed = ma.ed.reshape(ma.kpar, 25)[:, :12]
+    rows = [set() for _ in range(p.nb)]
+    for cell in ed:
+        for a, b in itertools.permutations(cell, 2):
+            if b < a:
+                rows[a].add(b)
+    e = ma.edges.reshape(-1, 2)
+    dirn = np.where(e[:, 1] - e[:, 0] == 1, 0, 1)
+    nxn, nyn, _ = tiny_model.shape_nodes
+    dirn = np.select([e[:, 1] - e[:, 0] == 1, e[:, 1] - e[:, 0] == nxn], [0, 1], 2)
+    for i in range(p.nb):
+        cols = p.jg[p.ig[i]:p.ig[i + 1]]
+        assert list(cols) == sorted(rows[i])
+        sizes = np.diff(p.ijg)[p.ig[i]:p.ig[i + 1]]
+        np.testing.assert_array_equal(sizes, np.where(dirn[cols] == dirn[i], 2, 1))
+    np.testing.assert_array_equal(np.diff(p.idi), 2)
+
+
+# ---- element matrices ----------------------------------------------------------------------
+
+_POS = {0: [(-1, -1), (1, -1), (-1, 1), (1, 1)],   # x-edges at (y, z)
+        1: [(-1, -1), (-1, 1), (1, -1), (1, 1)],   # y-edges at (x, z)
+        2: [(-1, -1), (1, -1), (-1, 1), (1, 1)]}   # z-edges at (x, y)
+_OTHER = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+
+
+def _quadrature_matrices(hx, hy, hz):
+    """Independent evaluation: explicit basis and np.cross curls, 3-point Gauss per axis."""
+    h = np.array([hx, hy, hz])
+    g, w = np.polynomial.legendre.leggauss(3)
+    X = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1)  # reference coords
+    W = np.einsum("i,j,k->ijk", w, w, w).ravel() * np.prod(h) / 8.0
+    phis, curls = [], []
+    for d in range(3):
+        k1, k2 = _OTHER[d]
+        for s1, s2 in _POS[d]:
+            f = (2.0 / h[d]) * 0.25 * (1 + s1 * X[k1]) * (1 + s2 * X[k2])
+            phi = np.zeros((3, X.shape[1]))
+            phi[d] = f
+            grad = np.zeros((3, X.shape[1]))
+            grad[k1] = (2.0 / h[d]) * 0.25 * s1 * (2.0 / h[k1]) * (1 + s2 * X[k2])
+            grad[k2] = (2.0 / h[d]) * 0.25 * s2 * (2.0 / h[k2]) * (1 + s1 * X[k1])
+            e = np.zeros(3)
+            e[d] = 1.0
+            curl = np.cross(grad.T, e).T
+            phis.append(phi)
+            curls.append(curl)
+    P = np.array(phis)
+    Cu = np.array(curls)
+    B = np.einsum("iaq,jaq,q->ij", Cu, Cu, W)
+    Cm = np.einsum("iaq,jaq,q->ij", P, P, W)
+    return B, Cm
+
+
+@pytest.mark.parametrize("h", [(1.0, 2.0, 3.0), (50.0, 70.0, 13.0), (100.0, 100.0, 537.8)])
+def test_element_matrices_vs_quadrature(M, h):
+    B, Cm = M.element_matrices(*h)
+    Bq, Cq = _quadrature_matrices(*h)
+    np.testing.assert_allclose(B, Bq, rtol=1e-12, atol=1e-12 * np.abs(Bq).max())
+    np.testing.assert_allclose(Cm, Cq, rtol=1e-12, atol=1e-12 * np.abs(Cq).max())
+    assert np.allclose(B, B.T) and np.allclose(Cm, Cm.T)
+
+
+def test_element_reproduces_linear_field(M):
+    """E = (-y, x, 0) lies in the edge space: int |curl E|^2 = 4V and int |E|^2 = V (hx^2+hy^2)/12
+    on a brick centred at the origin (DOF = tangential field * h/2)."""
+    hx, hy, hz = 3.0, 5.0, 2.0
+    B, Cm = M.element_matrices(hx, hy, hz)
+    d = np.zeros(12)
+    for k, (sy, sz) in enumerate(_POS[0]):
+        d[k] = -(sy * hy / 2) * hx / 2          # E_x = -y at y = +-hy/2
+    for k, (sx, sz) in enumerate(_POS[1]):
+        d[4 + k] = (sx * hx / 2) * hy / 2       # E_y = x at x = +-hx/2
+    V = hx * hy * hz
+    assert d @ B @ d == pytest.approx(4 * V, rel=1e-13)
+    assert d @ Cm @ d == pytest.approx(V * (hx ** 2 + hy ** 2) / 12, rel=1e-13)
+
+
+def test_gradients_in_curl_curl_kernel(M, tiny_model):
+    """K G^T phi = 0 for any nodal phi vanishing on the boundary, with G^T = (-1/2, +1/2) on an
+    edge's (first, second) node: pins the edge-DOF scaling and the G sign convention."""
+    import scipy.sparse as sp
+
+    p, kd, kg, sd, sg = M.assemble_ks(tiny_model)
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+    L = sp.csr_matrix((kg, (rows, p.jg)), shape=(p.nb, p.nb))
+    K = (L + L.T + sp.diags(kd)).tocsr()
+    ma = M.mesh_arrays(tiny_model)
+    rng = np.random.default_rng(0)
+    phi = rng.standard_normal(ma.kuzlov)
+    phi[ma.is_node_bound == 1] = 0.0
+    e = ma.edges.reshape(-1, 2)
+    v = 0.5 * (phi[e[:, 1]] - phi[e[:, 0]])
+    r = K @ v
+    assert np.abs(r).max() <= 1e-12 * abs(K).max() * np.abs(v).max() * 40
+    # and the mass part does not vanish on gradients (S is positive definite inside)
+    S = sp.csr_matrix((sg, (rows, p.jg)), shape=(p.nb, p.nb))
+    S = (S + S.T + sp.diags(sd)).tocsr()
+    assert v @ (S @ v) > 0
+
+
+def test_assemble_equals_k_plus_i_omega_s(M, tiny_model):
+    s = M.assemble(tiny_model, 0.37, 1)
+    p, kd, kg, sd, sg = M.assemble_ks(tiny_model)
+    w = M.omega_of(0.37)
+    A = M.sparse_matrix(s)
+    import scipy.sparse as sp
+
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+    K = sp.csr_matrix((kg, (rows, p.jg)), shape=(p.nb, p.nb))
+    S = sp.csr_matrix((sg, (rows, p.jg)), shape=(p.nb, p.nb))
+    B = (K + K.T + sp.diags(kd)) + 1j * w * (S + S.T + sp.diags(sd))
+    D = (A - B).tocoo()
+    assert np.abs(D.data).max(initial=0) <= 1e-12 * np.abs(A.data).max()
+    # S vanishes on orthogonal-edge blocks
+    assert np.all(sg[np.diff(p.ijg) == 1] == 0)
+
+
+@pytest.mark.parametrize("alpha", [0, 1])
+def test_rhs_only_assembly_is_identical(M, small_model, alpha):
+    s = M.assemble(small_model, 0.7, alpha)
+    np.testing.assert_array_equal(M.assemble_rhs(small_model, 0.7, alpha), s.pr)
+
+
+def test_dirichlet_rows_and_rhs_support(M, tiny_model):
+    ma = M.mesh_arrays(tiny_model)
+    e = ma.edges.reshape(-1, 2)
+    bnd = (ma.is_node_bound[e[:, 0]] == 1) & (ma.is_node_bound[e[:, 1]] == 1)
+    nxn = tiny_model.shape_nodes[0]
+    dirn = np.select([e[:, 1] - e[:, 0] == 1, e[:, 1] - e[:, 0] == nxn], [0, 1], 2)
+    for alpha, pol_dir in [(1, 0), (0, 1)]:
+        s = M.assemble(tiny_model, 2.0, alpha)
+        A = M.sparse_matrix(s).toarray()
+        for i in np.flatnonzero(bnd):
+            row = A[i].copy()
+            assert row[i] == 1.0
+            row[i] = 0
+            assert not row.any() and not A[:, i][np.arange(s.nb) != i].any()
+        b = M.to_complex(s.pr)
+        nz = np.flatnonzero(b)
+        assert len(nz) > 0 and not bnd[nz].any()
+        assert np.all(dirn[nz] == pol_dir)  # the normal field drives only edges along it
+    # sigma == sigma0 everywhere: zero right-hand side
+    flat = M.Model(tiny_model.x, tiny_model.y, tiny_model.z, tiny_model.sigma0.copy(),
+                   tiny_model.sigma0.copy(), tiny_model.layer_top, tiny_model.layer_sigma)
+    assert not M.assemble(flat, 1.0, 1).pr.any()
+
+
+# ---- 1-D normal field ------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nu,sig", [(1.0, 0.01), (100.0, 0.1), (1e-3, 1e-3)])
+def test_harm1d_halfspace_analytic(M, nu, sig):
+    """-(1/mu) A'' + i w sigma A = 0 below z = 0 with (1/mu) A'(0) = 1: A = (mu/k) e^{kz},
+    k = sqrt(i w mu sigma), Re k > 0. Checks the sign convention A0 = usin + i*ucos."""
+    m = M.Model(np.array([0.0, 1.0]), np.array([0.0, 1.0]), np.array([-1.0, 0.0]),
+                np.array([sig]), np.array([sig]), np.array([0.0]), np.array([sig]))
+    z, us, uc = M.harm1d(m, nu)
+    mu = 4e-7 * np.pi
+    k = np.sqrt(1j * 2 * np.pi * nu * mu * sig)
+    delta = 1 / k.real
+    sel = (z > -2 * delta) & (z <= 0)
+    exact = mu / k * np.exp(k * z[sel])
+    got = us[sel] + 1j * uc[sel]
+    err = np.abs(got - exact).max() / np.abs(exact).max()
+    assert err < 2e-3, err
+    assert z[0] == -1e7 and z[-1] == 0.0 and np.all(np.diff(z) > 0)
+
+
+def test_harm1d_mesh_rule(M):
+    """1-D nodes: steps of 1e-2 m growing x1.02 from the surface, layer interfaces inserted,
+    the last node at -1e7 (Mesh_1d_mtz::Gen_1d_mesh)."""
+    m = M.Model(np.array([0.0, 1.0]), np.array([0.0, 1.0]), np.array([-1.0, 0.0]),
+                np.array([0.01]), np.array([0.01]), np.array([-1000.0, 0.0]), np.array([0.1, 0.01]))
+    z, _, _ = M.harm1d(m, 1.0)
+    assert -1000.0 in z and z[0] == -1e7 and z[-1] == 0.0
+    zr = z[::-1]
+    steps = -np.diff(zr[:20])
+    np.testing.assert_allclose(steps, 1e-2 * 1.02 ** np.arange(19), rtol=1e-9)
+
+
+# ---- files ---------------------------------------------------------------------------------
+
+def test_files_roundtrip(M, tiny_model, tmp_path):
+    s = M.assemble(tiny_model, 1.0, 0)
+    d = M.write_cocr_fp_dir(tmp_path / "Nu1_Y", tiny_model, s)
+    r = M.read_block_system(d)
+    for f in ["ig", "jg", "idi", "ijg", "di", "gg", "pr"]:
+        np.testing.assert_array_equal(getattr(r, f), getattr(s, f))
+    assert (d / "kuslau").read_text().split() == [str(2 * s.nb), "1.000000e-08", "10000"]
+    mats = np.loadtxt(d / "sig3d")
+    assert mats[0, 1] == 1e-8 and int(mats[-1, 0]) == len(mats)

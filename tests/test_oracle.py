@@ -1,0 +1,160 @@
+"""Pinning the oracle (FP64 CPU restatement of COCR_FP) against independent checks.
+
+The reference ships no tests or golden vectors for this path and does not compile unmodified
+here, so every function of the oracle is pinned by mathematics: dense products, closed forms,
+brute-force construction, an independent textbook recurrence and the smoothing invariant.
+"""
+import numpy as np
+import pytest
+
+from helpers import brute_force_g, dense_preconditioner, identity_system, textbook_pcocr
+
+
+@pytest.fixture(scope="module")
+def tiny(M, tiny_model):
+    s = M.assemble(tiny_model, 3.0, 1)
+    ma = M.mesh_arrays(tiny_model)
+    A = M.sparse_matrix(s)
+    G = brute_force_g(ma, s.nb)
+    return s, ma, A, G
+
+
+@pytest.mark.parametrize("threads", [1, 3])
+def test_spmv_matches_dense(O, M, tiny, threads):
+    s, ma, A, G = tiny
+    x = np.random.default_rng(1).standard_normal(2 * s.nb)
+    y = O.spmv(s, x, threads)
+    ref = A @ M.to_complex(x)
+    np.testing.assert_allclose(M.to_complex(y), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    assert abs(A - A.T).max() == 0  # complex-symmetric, no conjugation
+
+
+def test_scal_cmplx_is_unconjugated(O, M):
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(200), rng.standard_normal(200)
+    want = np.sum(M.to_complex(x) * M.to_complex(y))
+    for t in (1, 4):
+        got = O.scal_cmplx(x, y, t)
+        assert abs(got - want) <= 1e-13 * abs(want)
+
+
+def test_gradient_matrix_bruteforce(O, M, tiny):
+    s, ma, A, G = tiny
+    d = O.Dfp(ma)
+    ip, jp, gv = d.g_csr()
+    import scipy.sparse as sp
+
+    Go = sp.csr_matrix((gv, jp, ip), shape=(d.m, s.nb))
+    assert d.m == G.shape[0]
+    assert abs(Go - G).max() == 0
+
+
+def test_diagonals_vs_dense(O, M, tiny):
+    s, ma, A, G = tiny
+    d = O.Dfp(ma, s)
+    da, dn = d.diagonals()
+    _, da_ref, dn_ref = dense_preconditioner(A, G)
+    np.testing.assert_allclose(M.to_complex(da), da_ref, rtol=1e-13)
+    np.testing.assert_allclose(M.to_complex(dn), dn_ref, rtol=1e-11)
+
+
+@pytest.mark.parametrize("threads", [1, 2])
+def test_preconditioner_closed_form(O, M, tiny, threads):
+    s, ma, A, G = tiny
+    d = O.Dfp(ma, s)
+    apply, _, _ = dense_preconditioner(A, G)
+    x = np.random.default_rng(3).standard_normal(2 * s.nb)
+    got = M.to_complex(d.apply(x, threads))
+    ref = apply(M.to_complex(x))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.fixture(scope="module")
+def earth(M):
+    """Air-free model: well-conditioned enough that two correct COCR implementations agree to
+    rounding over many iterations (with air at 1e-8 S/m, 1e-16 differences grow to 1e-8 within
+    three iterations, which no comparison of iterates can resolve)."""
+    m = M.random_model(6, 6, 6, seed=3, h=100.0, n_pad=1, n_air=0, name="earth")
+    s = M.assemble(m, 3.0, 1)
+    ma = M.mesh_arrays(m)
+    A = M.sparse_matrix(s)
+    return s, ma, A, brute_force_g(ma, s.nb)
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 15, 30])
+def test_iterates_match_textbook_pcocr(O, M, earth, k):
+    """x after k iterations (maxiter = k stops with REQ_MAXITER) equals the k-th iterate of an
+    independent textbook preconditioned COCR with dense operators."""
+    s, ma, A, G = earth
+    apply, _, _ = dense_preconditioner(A, G)
+    b = M.to_complex(s.pr)
+    xs = textbook_pcocr(A, apply, b, k)
+    x, st, _ = O.pcocr(s, ma, eps=1e-30, maxiter=k)
+    assert st["req"] == -3 and st["iter"] == k + 1
+    got = M.to_complex(x)
+    assert np.linalg.norm(got - xs[-1]) <= 1e-11 * np.linalg.norm(xs[-1])
+
+
+def test_first_iterate_with_air(O, M, tiny):
+    """With air cells the first iterate still agrees to rounding."""
+    s, ma, A, G = tiny
+    apply, _, _ = dense_preconditioner(A, G)
+    xs = textbook_pcocr(A, apply, M.to_complex(s.pr), 1)
+    x, st, _ = O.pcocr(s, ma, eps=1e-30, maxiter=1)
+    assert np.linalg.norm(M.to_complex(x) - xs[0]) <= 1e-12 * np.linalg.norm(xs[0])
+
+
+@pytest.mark.parametrize("which,tol", [("earth", 1e-9), ("tiny", 1e-6)])
+def test_converges_to_direct_solution(O, M, which, tol, request):
+    s, ma, A, G = request.getfixturevalue(which)
+    import scipy.sparse.linalg as spl
+
+    b = M.to_complex(s.pr)
+    xd = spl.spsolve(A.tocsc(), b)
+    x, st, tr = O.pcocr(s, ma, eps=1e-12, maxiter=5000, trace=True)
+    assert st["req"] == 0 and st["residual_rel"] < 1e-12
+    assert np.linalg.norm(M.to_complex(x) - xd) <= tol * np.linalg.norm(xd)
+    # request accounting: one SpMV for r0, one per iteration, one more unless it stopped
+    assert st["n_precond"] == st["iter"] + 1
+    assert st["n_spmv"] in (st["iter"] + 1, st["iter"] + 2)
+
+
+def test_mrs_invariant_and_monotone(O, M, c1_system, c1_mesh):
+    """The smoothed residual s_mrs equals b - A y_mrs, so the reported residualRel is
+    ||b - A y|| / ||b||; the smoothed residual never increases."""
+    s = c1_system
+    d = O.Dfp(c1_mesh)
+    x, y, st, tr = O.pcocr_with_mrs(s, d, eps=1e-8)
+    A = M.sparse_matrix(s)
+    b = M.to_complex(s.pr)
+    rel_y = np.linalg.norm(b - A @ M.to_complex(y)) / np.linalg.norm(b)
+    assert st["req"] == 0
+    assert abs(rel_y - st["residual_rel"]) <= 1e-3 * st["residual_rel"] + 1e-12
+    assert np.all(np.diff(tr) <= 1e-15)
+    # the returned x is the raw COCR iterate (not y): its residual differs from the MRS one
+    rel_x = np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b)
+    assert rel_x < 5e-8 and rel_x != pytest.approx(rel_y, rel=1e-6)
+
+
+def test_special_cases(O, M, tiny):
+    s, ma, A, G = tiny
+    # b = 0: x0 is the solution (REQ_X0_NULL), x stays 0
+    z = M.BlockSystem(s.nb, s.ig, s.jg, s.idi, s.ijg, s.di, s.gg, np.zeros_like(s.pr))
+    x, st, _ = O.pcocr(z, ma)
+    assert st["req"] == -1 and not x.any() and st["iter"] == 0
+    # maxiter = 1: REQ_MAXITER after one iteration
+    x, st, _ = O.pcocr(s, ma, maxiter=1)
+    assert st["req"] == -3 and st["iter"] == 2 and x.any()
+
+
+def test_div_by_zero(O, M):
+    m, ma, s = identity_system(M)
+    x, st, _ = O.pcocr(s, ma)
+    assert st["req"] == -2 and st["iter"] == 1
+
+
+def test_thread_count_only_changes_rounding(O, M, c1_system, c1_mesh):
+    x1, s1, _ = O.pcocr(c1_system, c1_mesh, nthreads=1)
+    x4, s4, _ = O.pcocr(c1_system, c1_mesh, nthreads=4)
+    assert abs(s1["iter"] - s4["iter"]) <= 3
+    assert np.linalg.norm(x1 - x4) <= 1e-5 * np.linalg.norm(x1)
