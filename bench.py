@@ -37,14 +37,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def sync():
-    try:
-        import torch
+_hip = None
 
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+
+def sync():
+    """Device-wide synchronisation (hipDeviceSynchronize) around the timed region."""
+    global _hip
+    if _hip is None:
+        import ctypes
+
+        _hip = ctypes.CDLL("libamdhip64.so")
+    _hip.hipDeviceSynchronize()
 
 
 def build_workload(args):

@@ -14,13 +14,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-def gpu_available() -> bool:
-    try:
-        import torch
+_gpu = None
 
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+
+def gpu_available() -> bool:
+    """HIP device count through the runtime library (no torch import)."""
+    global _gpu
+    if _gpu is None:
+        try:
+            import ctypes
+
+            hip = ctypes.CDLL("libamdhip64.so")
+            n = ctypes.c_int(0)
+            _gpu = hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+        except OSError:
+            _gpu = False
+    return _gpu
 
 
 @pytest.fixture(scope="session")
