@@ -79,6 +79,7 @@ struct GView {
 // complex division with scaling (Smith), used for alpha, beta and reciprocals
 __host__ __device__ inline void cdiv(double ar, double ai, double br, double bi, double& qr,
                                      double& qi) {
+#pragma clang fp contract(off)
   if (fabs(br) >= fabs(bi)) {
     const double r = bi / br, d = br + bi * r;
     qr = (ar + ai * r) / d;
@@ -195,6 +196,7 @@ __global__ void k_fill(T2* v, long long n) {
 template <class T, int SP>
 __global__ void k_diag_a(const double* da, const double* db, const double* omega,
                          typename Vec2<T>::type* out, int nb) {
+#pragma clang fp contract(off)
   const long long n = (long long)nb * SP;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -207,11 +209,39 @@ __global__ void k_diag_a(const double* da, const double* db, const double* omega
   }
 }
 
-// D_N^-1 = 1 / (G A G^T)_ii from the FP64 matrix (FoldedPreconditioner.cpp:527-568): for every
-// edge j of node i, row j of A is scanned for the node's other edges.
+// value of A(j, c) = a + i*omega*b in row j of the sliced ELL (0 when absent)
+__device__ inline bool ell_lookup(const EllView<double>& A, int j, int c, double w, double& vr,
+                                  double& vi) {
+  const int slice = j >> 6, lane = j & 63;
+  const int oc = A.offc[slice], ncs = A.nc[slice];
+  for (int q = 0; q < ncs; ++q) {
+    const int idx = oc + q * 64 + lane;
+    if (A.colc[idx] == c) {
+      vr = A.ac[idx];
+      vi = w * A.bc[idx];
+      return true;
+    }
+  }
+  const int orr = A.offr[slice], nrs = A.nr[slice];
+  for (int q = 0; q < nrs; ++q) {
+    const int idx = orr + q * 64 + lane;
+    if (A.colr[idx] == c) {
+      vr = A.ar[idx];
+      vi = 0.0;
+      return true;
+    }
+  }
+  return false;
+}
+
+// D_N^-1 = 1 / (G A G^T)_ii from the FP64 matrix (FoldedPreconditioner.cpp:527-568). The sums run
+// in the reference's order (for every edge j of node i ascending: the entries of row j in
+// ascending column order, then times G_ij) without fused multiply-adds. In air cells the real part
+// of (G A G^T)_ii is the rounding residue of G K G^T = 0, so this order is what fixes its value.
 template <class T, int SP>
 __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega,
                          typename Vec2<T>::type* out) {
+#pragma clang fp contract(off)
   const long long n = (long long)G.m * SP;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -220,37 +250,27 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
     const double w = omega[s];
     int ecol[8];
     double eval[8];
-    const int wg = G.wg < 8 ? G.wg : 8;
-    for (int k = 0; k < wg; ++k) {
-      ecol[k] = G.gcol[(size_t)k * G.m + i];
-      eval[k] = G.gval[(size_t)k * G.m + i];
+    int wg = 0;
+    for (int k = 0; k < G.wg && k < 8; ++k) {
+      const double g = G.gval[(size_t)k * G.m + i];
+      if (g == 0.0) continue;  // padding of shorter rows
+      ecol[wg] = G.gcol[(size_t)k * G.m + i];
+      eval[wg] = g;
+      ++wg;
     }
     double dr = 0.0, di = 0.0;
     for (int kj = 0; kj < wg; ++kj) {
-      const double gj = eval[kj];
-      if (gj == 0.0) continue;
       const int j = ecol[kj];
-      const int slice = j >> 6, lane = j & 63;
       double sr = 0.0, si = 0.0;
-      const int oc = A.offc[slice], ncs = A.nc[slice];
-      for (int q = 0; q < ncs; ++q) {
-        const int idx = oc + q * 64 + lane;
-        const int c = A.colc[idx];
-        for (int kk = 0; kk < wg; ++kk)
-          if (ecol[kk] == c && eval[kk] != 0.0) {
-            sr += A.ac[idx] * eval[kk];
-            si += w * A.bc[idx] * eval[kk];
-          }
+      for (int kk = 0; kk < wg; ++kk) {  // ascending edge = ascending column of row j
+        double vr, vi;
+        if (ell_lookup(A, j, ecol[kk], w, vr, vi)) {
+          sr += vr * eval[kk];
+          si += vi * eval[kk];
+        }
       }
-      const int orr = A.offr[slice], nrs = A.nr[slice];
-      for (int q = 0; q < nrs; ++q) {
-        const int idx = orr + q * 64 + lane;
-        const int c = A.colr[idx];
-        for (int kk = 0; kk < wg; ++kk)
-          if (ecol[kk] == c && eval[kk] != 0.0) sr += A.ar[idx] * eval[kk];
-      }
-      dr += sr * gj;
-      di += si * gj;
+      dr += sr * eval[kj];
+      di += si * eval[kj];
     }
     double qr, qi;
     cdiv(1.0, 0.0, dr, di, qr, qi);
