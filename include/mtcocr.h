@@ -70,8 +70,8 @@ const char* mtcocr_error_string(int code);
 int mtcocr_create(int device, mtcocr_handle** out);
 int mtcocr_destroy(mtcocr_handle* h);
 
-/* One system in the reference's block layout (uploads a device copy in the solver's own layout,
- * FP64 and FP32). */
+/* One system in the reference's block layout (uploads an FP64 device copy in the solver's own
+ * sliced layout). */
 int mtcocr_set_matrix_block2x2(mtcocr_handle* h, int nb, const int* ig, const int* jg,
                                const int* idi, const int* ijg, const double* di,
                                const double* gg);
@@ -124,9 +124,10 @@ int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi,
                             const int* jg_t, const double* gg_t, const int* is_node_bound,
                             mtcocr_stats* stats);
 
-/* Measurement hook: runs the SpMV kernel `reps` times on the handle's stream (precision 0 FP64,
- * 1 FP32; nsys systems of the current batch layout, 1 for a block2x2 matrix) and returns the
- * average kernel time from HIP events and the algorithmic bytes moved per launch. */
+/* Measurement hook: runs the SpMV kernel `reps` times on the handle's stream (precision 0: FP64
+ * vectors, 1: FP32 vectors; the matrix is FP64 in both) over the nsys systems of the current
+ * batch and returns the average kernel time from HIP events and the algorithmic bytes moved per
+ * launch (matrix once, x and y once per system). */
 int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, double* avg_ms,
                       double* bytes_per_launch);
 

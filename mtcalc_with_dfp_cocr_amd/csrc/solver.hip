@@ -283,7 +283,7 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
 // ||r||^2, plus the MRS initialisation ym = x, sm = r of MRS.cpp:91-97). MODE 2 is MODE 1 gated
 // by the outer (refinement) status instead of the COCR status.
 template <class T, int SP, int MODE>
-__global__ __launch_bounds__(kBlock) void k_spmv(EllView<T> A, const double* omega,
+__global__ __launch_bounds__(kBlock) void k_spmv(EllView<double> A, const double* omega,
                                                  const typename Vec2<T>::type* __restrict__ x,
                                                  typename Vec2<T>::type* __restrict__ y,
                                                  const typename Vec2<T>::type* __restrict__ b,
@@ -291,6 +291,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<T> A, const double* ome
                                                  typename Vec2<T>::type* __restrict__ sm,
                                                  const SysScalars* __restrict__ sc,
                                                  const int* __restrict__ nrun, double* partials) {
+  // The matrix is always applied from its FP64 values with FP64 accumulation, also when the
+  // vectors are FP32: rounding K to FP32 breaks the exact discrete-gradient null space
+  // (G K = 0) that the folded preconditioner inverts in the near-insulating air, and COCR then
+  // stagnates. Rounding happens only when the product is stored.
   using T2 = typename Vec2<T>::type;
   if (*nrun == 0) return;
   const long long N = (long long)A.nb * SP;
@@ -304,19 +308,20 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<T> A, const double* ome
     if ((MODE == 2 ? sc[s].outer_status : sc[s].status) != ST_RUNNING) continue;
     const int row = int(t / SP);
     const int slice = row >> 6, lane = row & 63;
-    const T w = T(omega[s]);
-    T yr = 0, yi = 0;
+    const double w = omega[s];
+    double yr = 0, yi = 0;
     {
       const int oc = A.offc[slice], ncs = A.nc[slice];
 #pragma unroll 4
       for (int k = 0; k < ncs; ++k) {
         const int idx = oc + k * 64 + lane;
         const int c = A.colc[idx];
-        const T a = A.ac[idx];
-        const T bb = A.bc[idx] * w;
+        const double a = A.ac[idx];
+        const double bb = A.bc[idx] * w;
         const T2 xv = x[(size_t)c * SP + s];
-        yr += a * xv.x - bb * xv.y;
-        yi += a * xv.y + bb * xv.x;
+        const double xr = double(xv.x), xi = double(xv.y);
+        yr += a * xr - bb * xi;
+        yi += a * xi + bb * xr;
       }
     }
     {
@@ -325,25 +330,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<T> A, const double* ome
       for (int k = 0; k < nrs; ++k) {
         const int idx = orr + k * 64 + lane;
         const int c = A.colr[idx];
-        const T a = A.ar[idx];
+        const double a = A.ar[idx];
         const T2 xv = x[(size_t)c * SP + s];
-        yr += a * xv.x;
-        yi += a * xv.y;
+        yr += a * double(xv.x);
+        yi += a * double(xv.y);
       }
     }
     if (MODE == 0) {
       T2 o;
-      o.x = yr;
-      o.y = yi;
+      o.x = T(yr);
+      o.y = T(yi);
       y[t] = o;
       const T2 xs = x[t];
-      v[0] += double(yr) * double(xs.x) - double(yi) * double(xs.y);
-      v[1] += double(yr) * double(xs.y) + double(yi) * double(xs.x);
+      // the dot uses the stored (rounded) product, as the reference's ScalCmplx(z, s) does
+      v[0] += double(o.x) * double(xs.x) - double(o.y) * double(xs.y);
+      v[1] += double(o.x) * double(xs.y) + double(o.y) * double(xs.x);
     } else {
       const T2 bv = b[t];
       T2 o;
-      o.x = bv.x - yr;
-      o.y = bv.y - yi;
+      o.x = T(double(bv.x) - yr);
+      o.y = T(double(bv.y) - yi);
       y[t] = o;
       v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
       v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
@@ -818,7 +824,6 @@ struct mtcocr_handle {
   int64_t nnz_c = 0, nnz_r = 0;
   DevBuf offc, nc, offr, nr, colc, colr;
   DevBuf ac64, bc64, ar64, da64, db64;
-  DevBuf ac32, bc32, ar32;
   // gradient
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0;
@@ -836,7 +841,7 @@ struct mtcocr_handle {
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
-                           &db64, &ac32, &bc32, &ar32, &gcol, &gtcol, &gval64, &gtval64,
+                           &db64, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vda, &vdn,
                            &vx, &vb};
@@ -890,6 +895,7 @@ int pad_sp(int nsys) {
 
 template <class T>
 EllView<T> ell_view(mtcocr_handle* h) {
+  static_assert(std::is_same<T, double>::value, "the matrix is held in FP64 only");
   EllView<T> v;
   v.offc = h->offc.as<int>();
   v.nc = h->nc.as<int>();
@@ -897,15 +903,9 @@ EllView<T> ell_view(mtcocr_handle* h) {
   v.nr = h->nr.as<int>();
   v.colc = h->colc.as<int>();
   v.colr = h->colr.as<int>();
-  if constexpr (std::is_same<T, double>::value) {
-    v.ac = h->ac64.as<double>();
-    v.bc = h->bc64.as<double>();
-    v.ar = h->ar64.as<double>();
-  } else {
-    v.ac = h->ac32.as<float>();
-    v.bc = h->bc32.as<float>();
-    v.ar = h->ar32.as<float>();
-  }
+  v.ac = h->ac64.as<double>();
+  v.bc = h->bc64.as<double>();
+  v.ar = h->ar64.as<double>();
   v.nb = h->nb;
   return v;
 }
@@ -934,7 +934,7 @@ template <class T, int SP>
 struct Engine {
   using T2 = typename Vec2<T>::type;
   mtcocr_handle* h;
-  EllView<T> A;
+  EllView<double> A;
   GView<T> G;
   T2 *x, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *da, *dn;
   SysScalars* sc;
@@ -944,7 +944,7 @@ struct Engine {
   int nb, m, grid_e, grid_n;
 
   explicit Engine(mtcocr_handle* hh) : h(hh) {
-    A = ell_view<T>(h);
+    A = ell_view<double>(h);
     G = g_view<T>(h);
     nb = h->nb;
     m = h->m;
@@ -1304,10 +1304,6 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
       (rc = upload(h, h->db64, E.diag_b, st)))
     return rc;
   HIPC(hipStreamSynchronize(st));
-  if ((rc = upload_cast_float(h, h->ac32, E.ac, st)) ||
-      (rc = upload_cast_float(h, h->bc32, E.bc, st)) ||
-      (rc = upload_cast_float(h, h->ar32, E.ar, st)))
-    return rc;
   h->have_matrix = true;
   h->nsys = 0;
   h->work_prec = -1;
@@ -1556,13 +1552,13 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
   });
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
-  // algorithmic bytes: matrix values + column indices once, x read once and y written once per
-  // system, the slice descriptors, and the fused dot's re-read of x (served by the same lines)
-  const double es = (precision == MTCOCR_PREC_FP64) ? 8.0 : 4.0;
+  // algorithmic bytes: FP64 matrix values + int32 column indices once per launch, then x read
+  // once and y written once per system (FP64 or FP32 vectors), plus the slice descriptors
+  const double ev = (precision == MTCOCR_PREC_FP64) ? 8.0 : 4.0;
   const double nnzc = double(h->nnz_c), nnzr = double(h->nnz_r);
-  const double val = (h->ks_form ? 2.0 * es * nnzc : 2.0 * es * nnzc) + es * nnzr;
+  const double val = 2.0 * 8.0 * nnzc + 8.0 * nnzr;
   const double idx = 4.0 * (nnzc + nnzr);
-  const double vec = double(nsys) * 2.0 * (2.0 * es * h->nb);
+  const double vec = double(nsys) * 2.0 * (2.0 * ev * h->nb);
   const double desc = 16.0 * h->nslices;
   *bytes_per_launch = val + idx + vec + desc;
   return 0;
