@@ -165,10 +165,15 @@ __device__ inline void count_running(const SysScalars* sc, int nsp, int* nrun) {
 }
 
 // ------------------------------------------------------------------------------------------
-// element-wise complex helpers: z = x + a*y written exactly as Base_solver::Cmplx_axpy
+// element-wise complex helpers: z = x + a*y written exactly as Base_solver::Cmplx_axpy.
+// Contraction is off so that every kernel that evaluates the same update gets the same bits:
+// the preconditioner's node kernel recomputes a = z + beta*a from z and the old a instead of
+// reading the edge kernel's stored result, and in the air G*a is a tiny difference that D_N^-1
+// amplifies, so a one-ulp disagreement between the two copies of a (FP32) breaks COCR.
 
 template <class T2>
 __device__ inline T2 caxpy(T2 x, T2 y, double ar, double ai) {
+#pragma clang fp contract(off)
   using T = decltype(x.x);
   const T a_re = T(ar), a_im = T(ai);
   T2 r;
