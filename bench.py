@@ -5,16 +5,18 @@ unknowns), 50 m core cells, 8 padding cells per side growing x1.4, 6 air cells a
 seeded log10(rho) ~ U[0,3] smoothed by a 3-cell Gaussian; 8 log-spaced frequencies from 1e-3 to
 1e2 Hz x 2 polarizations = 16 systems sharing K and S (A = K + i*omega*S).
 
-A "step" solves all 16 systems to a relative residual of 1e-8:
-  mixed_batched  (headline) one batched launch sequence, complex FP32 inner COCR + FP64 iterative
-                 refinement, stop on the true FP64 residual ||b - A x|| / ||b|| < 1e-8
-  fp64_single    the plain FP64 GPU solver, one system at a time, the reference's iteration and
-                 MRS stop test (reported beside it as the comparison)
-value = wall seconds of one mixed_batched step / 16 (inputs resident in HBM, timed between
-device synchronisations). The roofline is that of the batched FP32 SpMV kernel, timed with HIP
-events on the solver's stream. The CPU baseline is the oracle (FP64 CPU restatement of the
-reference's COCR_FP) on one system of the same workload for a bounded number of iterations,
-extrapolated with the iteration count the FP64 GPU solver needed for that system.
+A "step" solves all 16 systems to the reference's stop test (MRS-smoothed relative residual
+< 1e-8) with the batched FP64 solver: one launch sequence over all systems, the matrix read once
+per iteration for the whole batch, systems retiring independently.
+value = wall seconds of one step / 16 (inputs resident in HBM, timed between device
+synchronisations). Reported beside it:
+  fp64_single    the plain FP64 GPU solver, one system at a time (the comparison baseline)
+  mixed_batched  FP32 inner COCR + FP64 refinement on the true residual, FP64 finish for the
+                 systems whose refinement stalls
+The roofline is that of the dominant kernel (the batched FP64 SpMV), timed with HIP events on
+the solver's stream. The CPU baseline is the oracle (FP64 CPU restatement of the reference's
+COCR_FP) on one system of the same workload for a bounded number of iterations, extrapolated
+with the iteration count the FP64 GPU solver needed for that system.
 """
 from __future__ import annotations
 
@@ -72,12 +74,20 @@ def cpu_baseline(M, m, ma, case, iters_needed, budget_iters, threads):
 
     f, a = case
     s = M.assemble(m, f, a)
-    x1, st1, _ = O.pcocr(s, ma, eps=1e-30, maxiter=1, nthreads=threads)
-    xn, stn, _ = O.pcocr(s, ma, eps=1e-30, maxiter=budget_iters, nthreads=threads)
+    _, st1, _ = O.pcocr(s, ma, eps=1e-30, maxiter=1, nthreads=threads)
+    _, stn, _ = O.pcocr(s, ma, eps=1e-30, maxiter=budget_iters, nthreads=threads)
     per_iter = (stn["seconds"] - st1["seconds"]) / max(1, budget_iters - 1)
     setup = max(0.0, st1["seconds"] - per_iter)
     est = setup + per_iter * iters_needed
     return est, per_iter, setup, stn["seconds"] + st1["seconds"]
+
+
+def timed(S, prec, args):
+    sync()
+    t0 = time.perf_counter()
+    rc, st = S.solve_resident(args.eps, args.maxiter, prec)
+    sync()
+    return time.perf_counter() - t0, rc, st
 
 
 def main():
@@ -93,9 +103,11 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--maxiter", type=int, default=20000)
     ap.add_argument("--fp64-systems", type=int, default=-1,
-                    help="how many systems the plain FP64 comparison solves (-1: all)")
-    ap.add_argument("--cpu-iters", type=int, default=60, help="oracle iterations for the CPU baseline")
+                    help="how many systems the one-at-a-time FP64 comparison solves (-1: all)")
+    ap.add_argument("--cpu-iters", type=int, default=150,
+                    help="oracle iterations for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--spmv-reps", type=int, default=20)
     args = ap.parse_args()
 
@@ -112,70 +124,77 @@ def main():
     S = Solver(0)
     S.set_matrix_ks(p, *ks)
     S.set_gradient(ma)
-    info = S.info()
-    log(f"device layout: {info}")
+    log(f"device layout: {S.info()}")
 
-    # ---- headline: mixed precision, all systems batched ---------------------------------
-    S.set_rhs(prs, om)
+    # ---- headline: FP64, all systems batched --------------------------------------------
     for w in range(args.warmup):
-        rc, st = S.solve_resident(args.eps, args.maxiter, "mixed")
-        log(f"warmup {w}: rc={rc} iters={[s['iter'] for s in st]}")
         S.set_rhs(prs, om)
+        dt, rc, st = timed(S, "fp64", args)
+        log(f"warmup {w}: {dt:.3f}s rc={rc}")
     times = []
     stats = None
     for k in range(args.steps):
         S.set_rhs(prs, om)
-        sync()
-        t0 = time.perf_counter()
-        rc, stats = S.solve_resident(args.eps, args.maxiter, "mixed")
-        sync()
-        times.append(time.perf_counter() - t0)
-        log(f"step {k}: {times[-1]:.3f}s rc={rc}")
-    x_mixed = S.get_solution()
+        dt, rc, stats = timed(S, "fp64", args)
+        times.append(dt)
+        log(f"step {k}: {dt:.3f}s rc={rc}")
+    x_batched = S.get_solution()
     ms_step = 1e3 * float(np.mean(times))
     value = ms_step / 1e3 / nsys
-    mixed_ok = all(s["req"] == 0 for s in stats) and max(s["true_rel_res"] for s in stats) < args.eps
-    log(f"mixed batched: {ms_step:.1f} ms/step, {value:.4f} s/system, ok={mixed_ok}, "
-        f"inner iters {[s['iter'] for s in stats]}, passes {[s['outer'] for s in stats]}")
+    batched_ok = all(q["req"] == 0 for q in stats)
+    log(f"fp64 batched: {ms_step:.1f} ms/step, {value:.4f} s/system, ok={batched_ok}, "
+        f"iters {[q['iter'] for q in stats]}")
+
+    # ---- mixed precision, batched ---------------------------------------------------------
+    mixed = None
+    if not args.no_mixed:
+        S.set_rhs(prs, om)
+        dt, rc, st = timed(S, "mixed", args)
+        xm = S.get_solution()
+        mixed = {"s_per_system": dt / nsys, "ok": all(q["req"] == 0 for q in st),
+                 "true_rel_res_max": max(q["true_rel_res"] for q in st),
+                 "inner_iters": [q["iter"] for q in st], "passes": [q["outer"] for q in st],
+                 "fp64_finish": [q["flags"] & 1 for q in st],
+                 "x_rel_diff_vs_fp64_max": float(max(np.linalg.norm(xm[k] - x_batched[k]) /
+                                                     np.linalg.norm(x_batched[k])
+                                                     for k in range(nsys)))}
+        log(f"mixed batched: {mixed}")
 
     # ---- comparison: plain FP64 GPU solver, one system at a time ----------------------------
     n64 = nsys if args.fp64_systems < 0 else min(nsys, args.fp64_systems)
     fp64 = []
     for k in range(n64):
         S.set_rhs(prs[k:k + 1], om[k:k + 1])
-        sync()
-        t0 = time.perf_counter()
-        rc, st = S.solve_resident(args.eps, args.maxiter, "fp64")
-        sync()
-        dt = time.perf_counter() - t0
+        dt, rc, st = timed(S, "fp64", args)
         x64 = S.get_solution()[0]
-        d = np.linalg.norm(x64 - x_mixed[k]) / np.linalg.norm(x64)
-        fp64.append({"f": cases[k][0], "pol": "X" if cases[k][1] else "Y", "s": dt,
-                     "iter": st[0]["iter"], "req": st[0]["req"],
-                     "true_rel_res": st[0]["true_rel_res"], "mixed_vs_fp64_rel": d})
+        d = np.linalg.norm(x64 - x_batched[k]) / np.linalg.norm(x64)
+        fp64.append({"f": float(cases[k][0]), "pol": "X" if cases[k][1] else "Y", "s": dt,
+                     "iter": st[0]["iter"], "batched_iter": stats[k]["iter"], "req": st[0]["req"],
+                     "true_rel_res": st[0]["true_rel_res"], "batched_vs_single_rel": d})
         log(f"fp64 system {k}: {fp64[-1]}")
     fp64_per_sys = float(np.mean([r["s"] for r in fp64])) if fp64 else None
 
-    # ---- SpMV roofline (batched FP32 kernel of the headline, and single-system FP64) --------
+    # ---- SpMV roofline ----------------------------------------------------------------------
     S.set_rhs(prs, om)
-    spmv_ms, spmv_bytes = S.bench_spmv("mixed", args.spmv_reps)
+    spmv_ms, spmv_bytes = S.bench_spmv("fp64", args.spmv_reps)
+    spmv32_ms, spmv32_bytes = S.bench_spmv("mixed", args.spmv_reps)
     S.set_rhs(prs[:1], om[:1])
     spmv64_ms, spmv64_bytes = S.bench_spmv("fp64", args.spmv_reps)
-    ach = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    ach64 = spmv64_bytes / (spmv64_ms * 1e-3) / 1e9
-    log(f"spmv fp32 x{nsys}: {spmv_ms:.3f} ms, {ach:.0f} GB/s; fp64 x1: {spmv64_ms:.3f} ms, "
-        f"{ach64:.0f} GB/s")
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    ach, ach32, ach64 = gbs(spmv_bytes, spmv_ms), gbs(spmv32_bytes, spmv32_ms), gbs(spmv64_bytes, spmv64_ms)
+    log(f"spmv fp64 x{nsys}: {spmv_ms:.3f} ms {ach:.0f} GB/s; fp32-vector x{nsys}: "
+        f"{spmv32_ms:.3f} ms {ach32:.0f} GB/s; fp64 x1: {spmv64_ms:.3f} ms {ach64:.0f} GB/s")
     S.close()
 
     # ---- CPU baseline -------------------------------------------------------------------
     cpu = None
     if not args.no_cpu and fp64:
         threads = max(1, min(os.cpu_count() or 1, 16) - 1)
-        k = min(range(len(fp64)), key=lambda i: abs(np.log10(fp64[i]["f"])))  # the ~1 Hz system
+        k = min(range(len(fp64)), key=lambda i: abs(np.log10(fp64[i]["f"])))  # nearest 1 Hz
         est, per_iter, setup, spent = cpu_baseline(M, m, ma, cases[k], fp64[k]["iter"],
                                                    args.cpu_iters, threads)
-        cpu = {"value": est, "unit": "s per system (time-to-1e-8, extrapolated)", "cores": threads,
-               "kind": "port",
+        cpu = {"value": est, "unit": "s per system (time-to-1e-8, extrapolated)",
+               "cores": threads, "kind": "port",
                "sample": (f"oracle FP64 COCR_FP restatement, {args.cpu_iters} iterations of the "
                           f"{cases[k][0]:.3g} Hz {'X' if cases[k][1] else 'Y'} system "
                           f"(nb={p.nb}) on {threads} threads: setup {setup:.2f}s + "
@@ -183,6 +202,7 @@ def main():
                           f"(the FP64 GPU count); {spent:.1f}s of CPU work")}
         log(f"cpu baseline: {cpu}")
 
+    spb = 1 << (nsys - 1).bit_length()
     out = {
         "metric": METRIC,
         "value": value,
@@ -194,25 +214,26 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64+f64 (FP32 complex inner COCR, FP64 refinement)",
+        "dtype": "f64 (complex128)",
         "data": "synthetic (seeded random conductivity model)",
-        "config": {"workload": f"mixed-precision batched COCR, {args.nx}x{args.ny}x{args.nz} cells "
-                               f"(nb={p.nb}), {args.nfreq} freqs 1e-3..1e2 Hz x 2 pol = {nsys} "
-                               f"systems, eps={args.eps:g}",
+        "config": {"workload": f"batched FP64 COCR + folded preconditioner, {args.nx}x{args.ny}x"
+                               f"{args.nz} cells (nb={p.nb}), {args.nfreq} freqs 1e-3..1e2 Hz x 2 "
+                               f"pol = {nsys} systems, MRS stop at {args.eps:g}",
                    "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_spmv<float,{1 << (nsys - 1).bit_length()},0> (batched FP32 SpMV)",
+                     "kernel": f"k_spmv<double,{spb},0> (batched FP64 SpMV)",
                      "algorithmic_bytes": spmv_bytes, "avg_ms": spmv_ms},
         "cpu_baseline": cpu,
         "fp64_gpu_per_system_s": fp64_per_sys,
         "speedup_vs_fp64_gpu": (fp64_per_sys / value) if fp64_per_sys else None,
+        "batched_ok": batched_ok,
+        "batched_iters": [q["iter"] for q in stats],
+        "mixed_batched": mixed,
+        "spmv_fp32vec_batched": {"achieved_gbs": ach32, "avg_ms": spmv32_ms,
+                                 "algorithmic_bytes": spmv32_bytes, "frac": ach32 / HBM_PEAK_GBS},
         "spmv_fp64_single": {"achieved_gbs": ach64, "avg_ms": spmv64_ms,
                              "algorithmic_bytes": spmv64_bytes, "frac": ach64 / HBM_PEAK_GBS},
-        "mixed_ok": mixed_ok,
-        "mixed_inner_iters": [s["iter"] for s in stats],
-        "mixed_passes": [s["outer"] for s in stats],
-        "mixed_true_rel_res_max": max(s["true_rel_res"] for s in stats),
         "fp64_systems": fp64,
     }
     print(json.dumps(out), flush=True)

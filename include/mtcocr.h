@@ -48,6 +48,8 @@ extern "C" {
 #define MTCOCR_ERR_STATE (-13) /* call order: matrix / gradient / rhs missing */
 #define MTCOCR_ERR_NOTCONV (-14) /* mixed precision: refinement stalled above the tolerance */
 
+#define MTCOCR_FLAG_FP64_FINISH 1 /* mixed: refinement stalled, finished by the FP64 solver */
+
 #define MTCOCR_PREC_FP64 0  /* plain FP64 COCR, the reference's iteration and stop test */
 #define MTCOCR_PREC_MIXED 1 /* complex FP32 inner COCR + FP64 iterative refinement */
 
@@ -57,7 +59,7 @@ typedef struct mtcocr_stats {
   int req;                /* request code of this system */
   int iter;               /* COCR iterations (mixed: sum of inner iterations) */
   int outer;              /* refinement passes (FP64 path: 1) */
-  int reserved;
+  int flags;              /* MTCOCR_FLAG_* */
   double residual_rel;    /* last MRS-smoothed relative residual (reference's residualRel) */
   double true_rel_res;    /* ||b - A x|| / ||b|| in FP64, recomputed on the device */
   double seconds;         /* device time of DFP setup + iterations (the reference's kit span) */

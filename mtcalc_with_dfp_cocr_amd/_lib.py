@@ -23,7 +23,7 @@ class MtcocrStats(C.Structure):
         ("req", C.c_int),
         ("iter", C.c_int),
         ("outer", C.c_int),
-        ("reserved", C.c_int),
+        ("flags", C.c_int),
         ("residual_rel", C.c_double),
         ("true_rel_res", C.c_double),
         ("seconds", C.c_double),
@@ -31,7 +31,7 @@ class MtcocrStats(C.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 # exported symbols and their signatures; tests check that every declaration of include/*.h

@@ -57,9 +57,10 @@ struct SysScalars {
   double zs_re, zs_im, zs1_re, zs1_im;
   double aw_re, aw_im, al_re, al_im, be_re, be_im;
   double r_old, res, rel, w, eps;
-  double scale, prev_rel, b_norm;
+  double scale, prev_rel, b_norm, best_rel;
   int mode, status, iter, total_iter;
-  int outer_status, outer, pad0, pad1;
+  int outer_status, outer, last_improve, stall_window;
+  int fallback, fallback_iter, pad0, pad1;
 };
 
 template <class T>
@@ -575,6 +576,8 @@ __global__ __launch_bounds__(kScalarThreads) void s_prologue(const double* part,
     const double rn = rr > 0.0 ? sqrt(rr) : 0.0;
     sc[s].res = rn;  // MRS init: residual = ||r||
     sc[s].rel = 1.0;
+    sc[s].best_rel = 1.0;
+    sc[s].last_improve = 0;
     sc[s].be_re = sc[s].be_im = 0.0;
     sc[s].iter = 0;
     if (rn < 1e-30) sc[s].status = MTCOCR_X0_NULL;
@@ -644,7 +647,18 @@ __global__ __launch_bounds__(kScalarThreads) void s_stop(const double* part, int
     if (sc[s].mode == MRS_BLEND) sc[s].res = sqrt(ss);
     else if (sc[s].mode == MRS_COPY) sc[s].res = ss > 0.0 ? sqrt(ss) : 0.0;
     sc[s].rel = sc[s].res / sc[s].r_old;
-    if (sc[s].rel < sc[s].eps) sc[s].status = MTCOCR_OK;
+    if (sc[s].rel < sc[s].eps) {
+      sc[s].status = MTCOCR_OK;
+    } else if (sc[s].stall_window > 0) {
+      // inner solves of the refinement stop when the smoothed residual has not halved within
+      // stall_window iterations (FP32 has reached its attainable accuracy)
+      if (sc[s].rel < 0.5 * sc[s].best_rel) {
+        sc[s].best_rel = sc[s].rel;
+        sc[s].last_improve = sc[s].iter;
+      } else if (sc[s].iter - sc[s].last_improve > sc[s].stall_window) {
+        sc[s].status = MTCOCR_MAXITER;
+      }
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) count_running(sc, SP, nrun);
@@ -718,7 +732,8 @@ __global__ __launch_bounds__(kScalarThreads) void s_outer(const double* part, in
         sc[s].rel = rel;
         if (rel < eps) {
           sc[s].outer_status = MTCOCR_OK;
-        } else if (!first && rel > 0.9 * sc[s].prev_rel) {
+        } else if (!first && rel > 0.1 * sc[s].prev_rel) {
+          // a refinement pass that gains less than 10x hands the system to the FP64 solver
           sc[s].outer_status = MTCOCR_ERR_NOTCONV;
         } else if (sc[s].outer >= max_outer) {
           sc[s].outer_status = MTCOCR_ERR_NOTCONV;
@@ -1102,14 +1117,19 @@ void set_eps_status(mtcocr_handle* h, std::vector<SysScalars>& hs, int nsys, int
   (void)h;
 }
 
+// Plain FP64 COCR on the systems of `mask` (all when null), warm-started from x64.
 template <int SP>
-int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
+int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
+               const std::vector<char>* mask = nullptr) {
   hipStream_t st = h->stream;
   int rc = alloc_work(h, MTCOCR_PREC_FP64, SP);
   if (rc) return rc;
   Engine<double, SP> E(h);
   std::vector<SysScalars> hs;
   set_eps_status(h, hs, h->nsys, SP, eps);
+  if (mask)
+    for (int s = 0; s < h->nsys; ++s)
+      if (!(*mask)[s]) hs[s].status = hs[s].outer_status = ST_INACTIVE;
   HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
   int one = 1;
   HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
@@ -1131,7 +1151,6 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
   HIPC(hipMemcpyAsync(h->x64.p, E.x, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
   HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
-  // true residual with all systems active
   std::vector<SysScalars> tmp(hs);
   for (int s = 0; s < SP; ++s) tmp[s].status = (s < h->nsys) ? ST_RUNNING : ST_INACTIVE;
   HIPC(hipMemcpyAsync(E.sc, tmp.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
@@ -1143,6 +1162,7 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
   HIPC(hipMemcpyAsync(partials.data(), E.part, partials.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
   for (int s = 0; s < h->nsys; ++s) {
+    if (mask && !(*mask)[s]) continue;
     double bb = 0, rr = 0;
     for (int b = 0; b < E.grid_e; ++b) {
       bb += partials[((size_t)b * SP + s) * 2];
@@ -1152,6 +1172,7 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
     o.req = hs[s].status == ST_RUNNING ? MTCOCR_MAXITER : hs[s].status;
     o.iter = hs[s].iter;
     o.outer = 1;
+    o.flags = 0;
     o.residual_rel = hs[s].rel;
     o.true_rel_res = bb > 0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
     o.seconds = ms * 1e-3;
@@ -1159,62 +1180,98 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
   return 0;
 }
 
+// Mixed precision: FP32 inner COCR (FP32 vectors and preconditioner, FP64 matrix) inside an
+// FP64 iterative refinement on the true residual. A system whose refinement pass gains less
+// than 10x (the inner FP32 solve has reached its attainable accuracy: at low frequencies the
+// galvanic part of the field lies below FP32 resolution) is finished by the FP64 solver,
+// warm-started from the refined iterate.
 template <int SP>
 int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
   hipStream_t st = h->stream;
   int rc = alloc_work(h, MTCOCR_PREC_MIXED, SP);
   if (rc) return rc;
-  Engine<float, SP> E(h);
-  EllView<double> A64 = ell_view<double>(h);
   std::vector<SysScalars> hs;
-  set_eps_status(h, hs, h->nsys, SP, eps);
-  HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
-  int one = 1;
-  int* nrun_outer = h->nrun2.as<int>();
-  HIPC(hipMemcpyAsync(nrun_outer, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  HIPC(hipStreamSynchronize(st));
-  const double inner_floor = 1e-5;
-  const int max_outer = 40;
-  HIPC(hipEventRecord(h->ev0, st));
-  E.setup_diagonals(st);
-  const long long nv = (long long)h->nb * SP;
-  const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
-  for (int pass = 0;; ++pass) {
-    if (pass > max_outer + 1) {
-      std::snprintf(g_err, sizeof(g_err), "refinement did not finish after %d passes", pass);
-      return MTCOCR_ERR_HIP;
-    }
-    // FP64 true residual r64 = b - A x64 on the systems still refining
-    k_spmv<double, SP, 2><<<E.grid_e, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
-                                                       h->r64.as<double2>(), h->b64.as<double2>(),
-                                                       nullptr, nullptr, E.sc, nrun_outer, E.part);
-    s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e, E.sc, nrun_outer, E.nrun, eps,
-                                              inner_floor, pass == 0 ? 1 : 0, max_outer);
-    HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    if (h->h_flags[0] == 0) break;
-    k_to_inner<SP><<<gv, 256, 0, st>>>(h->r64.as<double2>(), E.b, E.x, E.sc, h->nb);
-    E.prologue(st, maxiter);
-    HIPC(hipGetLastError());
-    rc = run_iterations<float, SP>(h, E, maxiter);
-    if (rc) return rc;
-    s_inner_done<SP><<<1, 64, 0, st>>>(E.sc);
-    k_from_inner<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), E.ym, E.sc, h->nb);
-  }
-  HIPC(hipEventRecord(h->ev1, st));
-  HIPC(hipEventSynchronize(h->ev1));
   float ms = 0.f;
-  HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
+  {
+    Engine<float, SP> E(h);
+    EllView<double> A64 = ell_view<double>(h);
+    set_eps_status(h, hs, h->nsys, SP, eps);
+    for (int s = 0; s < SP; ++s) hs[s].stall_window = 200;
+    HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
+    int one = 1;
+    int* nrun_outer = h->nrun2.as<int>();
+    HIPC(hipMemcpyAsync(nrun_outer, &one, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    const double inner_floor = 1e-5;
+    const int max_outer = 40;
+    HIPC(hipEventRecord(h->ev0, st));
+    E.setup_diagonals(st);
+    const long long nv = (long long)h->nb * SP;
+    const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
+    for (int pass = 0;; ++pass) {
+      if (pass > max_outer + 1) {
+        std::snprintf(g_err, sizeof(g_err), "refinement did not finish after %d passes", pass);
+        return MTCOCR_ERR_HIP;
+      }
+      // FP64 true residual r64 = b - A x64 on the systems still refining
+      k_spmv<double, SP, 2><<<E.grid_e, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
+                                                         h->r64.as<double2>(), h->b64.as<double2>(),
+                                                         nullptr, nullptr, E.sc, nrun_outer, E.part);
+      s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e, E.sc, nrun_outer, E.nrun, eps,
+                                                inner_floor, pass == 0 ? 1 : 0, max_outer);
+      HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      if (h->h_flags[0] == 0) break;
+      k_to_inner<SP><<<gv, 256, 0, st>>>(h->r64.as<double2>(), E.b, E.x, E.sc, h->nb);
+      E.prologue(st, maxiter);
+      HIPC(hipGetLastError());
+      rc = run_iterations<float, SP>(h, E, maxiter);
+      if (rc) return rc;
+      s_inner_done<SP><<<1, 64, 0, st>>>(E.sc);
+      k_from_inner<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), E.ym, E.sc, h->nb);
+    }
+    HIPC(hipEventRecord(h->ev1, st));
+    HIPC(hipEventSynchronize(h->ev1));
+    HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+  }
+  std::vector<char> fall(h->nsys, 0);
+  bool any = false;
   for (int s = 0; s < h->nsys; ++s) {
     mtcocr_stats& o = stats[s];
     o.req = hs[s].outer_status == ST_RUNNING ? MTCOCR_ERR_NOTCONV : hs[s].outer_status;
     o.iter = hs[s].total_iter;
     o.outer = hs[s].outer;
+    o.flags = 0;
     o.residual_rel = hs[s].rel;
     o.true_rel_res = hs[s].rel;
     o.seconds = ms * 1e-3;
+    if (o.req == MTCOCR_ERR_NOTCONV) {
+      fall[s] = 1;
+      any = true;
+    }
+  }
+  if (any) {
+    // FP64 finish; its MRS stop test is tightened so that the true residual meets eps
+    std::vector<mtcocr_stats> fs(h->nsys);
+    rc = solve_fp64<SP>(h, eps / 3.0, maxiter, fs.data(), &fall);
+    if (rc) return rc;
+    for (int s = 0; s < h->nsys; ++s) {
+      if (!fall[s]) continue;
+      mtcocr_stats& o = stats[s];
+      o.req = fs[s].req == MTCOCR_OK && fs[s].true_rel_res >= eps ? MTCOCR_ERR_NOTCONV : fs[s].req;
+      o.iter += fs[s].iter;
+      o.flags |= MTCOCR_FLAG_FP64_FINISH;
+      o.residual_rel = fs[s].residual_rel;
+      o.true_rel_res = fs[s].true_rel_res;
+    }
+    const double extra = [&] {
+      for (int s = 0; s < h->nsys; ++s)
+        if (fall[s]) return fs[s].seconds;
+      return 0.0;
+    }();
+    for (int s = 0; s < h->nsys; ++s) stats[s].seconds += extra;
   }
   return 0;
 }

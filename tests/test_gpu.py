@@ -6,8 +6,8 @@ Tolerances:
   full FP64 solve: iteration count within 2% (+2) of the oracle, MRS residual < eps, true
     residual <= 3e-8 (the reference's own returned x reaches 1.0e-8..2.9e-8), x within 1e-5
     relative of the oracle's x (rounding growth through COCR on a system with 1e-8 S/m air)
-  mixed precision: true FP64 residual < eps = 1e-8, x within 1e-6 of the FP64 GPU solution's
-    distance to the oracle, i.e. ||x_mixed - x_oracle|| <= 1e-5 ||x_oracle||
+  mixed precision: true FP64 residual < eps = 1e-8 (systems whose FP32 refinement stalls are
+    finished by the FP64 solver and flagged), ||x_mixed - x_oracle|| <= 1e-5 ||x_oracle||
 """
 import os
 import subprocess
@@ -97,7 +97,7 @@ def test_mixed_solve(S, O, M, c1_system, oracle_c1):
     xo, so = oracle_c1
     x, st = S.solve(c1_system.pr, eps=1e-8, maxiter=10000, precision="mixed")
     say(f"mixed {st}")
-    assert st["req"] == 0
+    assert st["req"] == 0 and st["true_rel_res"] < 1e-8
     A = M.sparse_matrix(c1_system)
     b = M.to_complex(c1_system.pr)
     true_rel = np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b)
