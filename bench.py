@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--tune-reps", type=int, default=5,
+                    help="launches per candidate shape in the autotuner")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,6 +127,15 @@ def main():
     S.set_matrix_ks(p, *ks)
     S.set_gradient(ma)
     log(f"device layout: {S.info()}")
+
+    # ---- launch shapes: autotuned on the batch before anything is timed -------------------
+    S.set_rhs(prs, om)
+    tune = {"batched_fp64": S.autotune("fp64", args.tune_reps)}
+    if not args.no_mixed:
+        tune["batched_fp32vec"] = S.autotune("mixed", args.tune_reps)
+    S.set_rhs(prs[:1], om[:1])
+    tune["single_fp64"] = S.autotune("fp64", args.tune_reps)
+    log(f"autotuned launch shapes: {tune}")
 
     # ---- headline: FP64, all systems batched --------------------------------------------
     for w in range(args.warmup):
@@ -219,7 +230,8 @@ def main():
         "config": {"workload": f"batched FP64 COCR + folded preconditioner, {args.nx}x{args.ny}x"
                                f"{args.nz} cells (nb={p.nb}), {args.nfreq} freqs 1e-3..1e2 Hz x 2 "
                                f"pol = {nsys} systems, MRS stop at {args.eps:g}",
-                   "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU"},
+                   "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU",
+                   "launch": tune},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_spmv<double,{spb},0> (batched FP64 SpMV)",

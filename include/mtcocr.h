@@ -16,6 +16,9 @@
  *                             COCR_FP processes (Code/CalcStarter/main.cpp:291-325) over one
  *                             shared sparsity, A_s = K + i*omega_s*S.
  *   mtcocr_bench_spmv         no counterpart (measurement hook for the SpMV kernel).
+ *   mtcocr_set_launch, mtcocr_get_launch, mtcocr_autotune
+ *                             no counterpart (launch shapes of the GPU kernels; the reference
+ *                             is a CPU OpenMP solver with no tunable launch).
  *
  * Conventions: all pointers are host pointers to caller-owned memory; indices are 0-based int32
  * (the reference decrements its 1-based files before the call, HarmVect3d.cpp:203-226); complex
@@ -132,6 +135,23 @@ int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi,
  * launch (matrix once, x and y once per system). */
 int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, double* avg_ms,
                       double* bytes_per_launch);
+
+/* Launch shapes: workgroup caps of the grid-stride SpMV, edge-streaming and node kernels
+ * (multiples of 8 in [8, 16384]; 0 keeps a value). set_launch applies to both precisions and
+ * every batch width; get_launch reports those of `precision` at the current batch width. Caps
+ * change only the reduction grouping of the dot products (results are deterministic for a fixed
+ * shape). Defaults 8192 / 2048 / 8192. */
+int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node);
+int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
+                      int* cap_node);
+
+/* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
+ * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV, node and edge
+ * kernels at caps 512..16384, `reps` launches each, in `precision`, and keeps the fastest cap of
+ * each family on the handle for this precision and batch width (padded system count). Outputs (any may be NULL): the chosen caps and the SpMV time in ms
+ * at its chosen cap. Writes only work buffers: resident right-hand sides and solutions stay. */
+int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
+                    int* cap_node, double* spmv_ms);
 
 /* Kernel-level hook for parity checks: y = op(x) on the current matrix, op 0 = SpMV (A x),
  * op 1 = folded preconditioner (D_A^-1 x + G^T D_N^-1 G x), in precision 0 (FP64) or 1 (FP32

@@ -127,6 +127,26 @@ class Solver:
                                         C.byref(ms), C.byref(by)), "mtcocr_bench_spmv")
         return ms.value, by.value
 
+    def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0) -> None:
+        """Workgroup caps of the SpMV / edge-streaming / node kernels (0 keeps a value)."""
+        check(self._L.mtcocr_set_launch(self._h, int(cap_spmv), int(cap_vec), int(cap_node)),
+              "mtcocr_set_launch")
+
+    def get_launch(self, precision: str = "fp64") -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(self._L.mtcocr_get_launch(self._h, PRECISION[precision], C.byref(a), C.byref(b),
+                                        C.byref(c)), "mtcocr_get_launch")
+        return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value}
+
+    def autotune(self, precision: str = "fp64", reps: int = 10) -> dict:
+        """Time the tunable kernels of the current batch at each candidate cap and keep the
+        fastest (needs a batch on the handle: set_rhs or a previous solve_batch)."""
+        a, b, c, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(self._L.mtcocr_autotune(self._h, PRECISION[precision], int(reps), C.byref(a),
+                                      C.byref(b), C.byref(c), C.byref(ms)), "mtcocr_autotune")
+        return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
+                "spmv_ms": ms.value}
+
     def apply(self, op: str, x, precision: str = "fp64"):
         """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""
         y = np.empty(2 * self.nb)

@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "host_prep.h"
@@ -41,7 +42,7 @@ namespace mtcocr {
 // small helpers
 
 constexpr int kBlock = 256;      // threads per block of the streaming kernels
-constexpr int kMaxBlocks = 1024; // cap of the grid-stride kernels (fixed reduction grid)
+constexpr int kMaxBlocks = 16384; // largest grid of a reduction kernel (size of the partials)
 constexpr int kScalarThreads = 1024;
 constexpr int kXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
 
@@ -94,9 +95,12 @@ __host__ __device__ inline void cdiv(double ar, double ai, double br, double bi,
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
-inline int grid_for(long long n) {
+// Grid of a grid-stride kernel over n elements: one workgroup per 256-element tile up to `cap`
+// workgroups (the launch shape the autotuner picks), a multiple of the XCD count.
+inline int grid_for(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
-  if (b > kMaxBlocks) b = kMaxBlocks;
+  if (cap > kMaxBlocks) cap = kMaxBlocks;
+  if (b > cap) b = cap;
   if (b < kXcd) b = kXcd;
   return round_up(int(b), kXcd);
 }
@@ -146,8 +150,11 @@ __device__ inline void reduce_partials(const double* part, int nblk, double* tot
   const int t = threadIdx.x;
   const int q = t % P, j = t / P;
   double acc = 0.0;
-  if (j < G)
+  if (j < G) {
+    // unrolled so that the loads of one thread are in flight together (up to 16384 blocks)
+#pragma unroll 8
     for (int b = j; b < nblk; b += G) acc += part[(size_t)b * P + q];
+  }
   tmp[t] = acc;
   __syncthreads();
   if (t < P) {
@@ -858,6 +865,23 @@ struct mtcocr_handle {
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
   int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
+  // launch shapes (workgroup caps of the grid-stride kernels) per vector precision (FP64,
+  // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
+  int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
+  static int sp_index(int sp) {
+    int k = 0;
+    while ((1 << k) < sp && k < 6) ++k;
+    return k;
+  }
+  void set_caps(int cs, int cv, int cn) {
+    for (int p = 0; p < 2; ++p)
+      for (int k = 0; k < 7; ++k) {
+        if (cs) cap_spmv[p][k] = cs;
+        if (cv) cap_vec[p][k] = cv;
+        if (cn) cap_node[p][k] = cn;
+      }
+  }
+  mtcocr_handle() { set_caps(8192, 2048, 8192); }
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
@@ -961,7 +985,7 @@ struct Engine {
   int* nrun;
   double* part;
   const double* omega;
-  int nb, m, grid_e, grid_n;
+  int nb, m, grid_s, grid_e, grid_n;
 
   explicit Engine(mtcocr_handle* hh) : h(hh) {
     A = ell_view<double>(h);
@@ -985,8 +1009,10 @@ struct Engine {
     nrun = h->nrun.as<int>();
     part = h->part.as<double>();
     omega = h->omega.as<double>();
-    grid_e = grid_for((long long)nb * SP);
-    grid_n = grid_for((long long)m * SP);
+    const int k = mtcocr_handle::sp_index(SP), q = std::is_same<T, float>::value ? 1 : 0;
+    grid_s = grid_for((long long)nb * SP, h->cap_spmv[q][k]);
+    grid_e = grid_for((long long)nb * SP, h->cap_vec[q][k]);
+    grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
   }
 
   void setup_diagonals(hipStream_t st) {
@@ -998,8 +1024,8 @@ struct Engine {
 
   // stages 1-4 of the request order: r = b - A x, X0 test, s = M^-1 r, p = a = w = 0, z = A s
   void prologue(hipStream_t st, int maxiter) {
-    k_spmv<T, SP, 1><<<grid_e, kBlock, 0, st>>>(A, omega, x, r, b, ym, sm, sc, nrun, part);
-    s_prologue<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    k_spmv<T, SP, 1><<<grid_s, kBlock, 0, st>>>(A, omega, x, r, b, ym, sm, sc, nrun, part);
+    s_prologue<SP><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun);
     k_pc_node<T, SP, false><<<grid_n, kBlock, 0, st>>>(G, r, nullptr, dn, help, sc, nrun);
     k_pc_edge_plain<T, SP><<<grid_e, kBlock, 0, st>>>(G, r, da, help, s, sc, nrun);
     const long long nv = (long long)nb * SP;
@@ -1007,9 +1033,9 @@ struct Engine {
     k_fill<T2><<<gf, 256, 0, st>>>(p, nv);
     k_fill<T2><<<gf, 256, 0, st>>>(a, nv);
     k_fill<T2><<<gf, 256, 0, st>>>(w, nv);
-    k_spmv<T, SP, 0><<<grid_e, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
+    k_spmv<T, SP, 0><<<grid_s, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
                                                 nrun, part);
-    s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun, maxiter);
+    s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
   }
 
   // one COCR iteration (stages 5-8)
@@ -1021,9 +1047,9 @@ struct Engine {
     s_mrs<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc);
     k_mrs2<T, SP><<<grid_e, kBlock, 0, st>>>(x, r, ym, sm, s, w, sc, nrun, part, nb);
     s_stop<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
-    k_spmv<T, SP, 0><<<grid_e, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
+    k_spmv<T, SP, 0><<<grid_s, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
                                                 nrun, part);
-    s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun, maxiter);
+    s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
   }
 };
 
@@ -1155,16 +1181,16 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   for (int s = 0; s < SP; ++s) tmp[s].status = (s < h->nsys) ? ST_RUNNING : ST_INACTIVE;
   HIPC(hipMemcpyAsync(E.sc, tmp.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  k_spmv<double, SP, 1><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.x, h->r64.as<double2>(), E.b,
+  k_spmv<double, SP, 1><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.x, h->r64.as<double2>(), E.b,
                                                      nullptr, nullptr, E.sc, E.nrun, E.part);
   HIPC(hipGetLastError());
-  std::vector<double> partials((size_t)E.grid_e * SP * 2);
+  std::vector<double> partials((size_t)E.grid_s * SP * 2);
   HIPC(hipMemcpyAsync(partials.data(), E.part, partials.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
   for (int s = 0; s < h->nsys; ++s) {
     if (mask && !(*mask)[s]) continue;
     double bb = 0, rr = 0;
-    for (int b = 0; b < E.grid_e; ++b) {
+    for (int b = 0; b < E.grid_s; ++b) {
       bb += partials[((size_t)b * SP + s) * 2];
       rr += partials[((size_t)b * SP + s) * 2 + 1];
     }
@@ -1214,10 +1240,10 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
         return MTCOCR_ERR_HIP;
       }
       // FP64 true residual r64 = b - A x64 on the systems still refining
-      k_spmv<double, SP, 2><<<E.grid_e, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
+      k_spmv<double, SP, 2><<<E.grid_s, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
                                                          h->r64.as<double2>(), h->b64.as<double2>(),
                                                          nullptr, nullptr, E.sc, nrun_outer, E.part);
-      s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e, E.sc, nrun_outer, E.nrun, eps,
+      s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_s, E.sc, nrun_outer, E.nrun, eps,
                                                 inner_floor, pass == 0 ? 1 : 0, max_outer);
       HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
@@ -1589,21 +1615,21 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
     if (precision == MTCOCR_PREC_FP64) {
       Engine<double, SP> E(h);
       k_fill<double2><<<gf, 256, 0, st>>>(E.s, nv);
-      k_spmv<double, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
+      k_spmv<double, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
                                                          nullptr, E.sc, E.nrun, E.part);
       HIPC(hipEventRecord(h->ev0, st));
       for (int i = 0; i < reps; ++i)
-        k_spmv<double, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
+        k_spmv<double, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
                                                            nullptr, nullptr, E.sc, E.nrun, E.part);
       HIPC(hipEventRecord(h->ev1, st));
     } else {
       Engine<float, SP> E(h);
       k_fill<float2><<<gf, 256, 0, st>>>(E.s, nv);
-      k_spmv<float, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
+      k_spmv<float, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
                                                         nullptr, E.sc, E.nrun, E.part);
       HIPC(hipEventRecord(h->ev0, st));
       for (int i = 0; i < reps; ++i)
-        k_spmv<float, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
+        k_spmv<float, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
                                                           nullptr, nullptr, E.sc, E.nrun, E.part);
       HIPC(hipEventRecord(h->ev1, st));
     }
@@ -1623,6 +1649,124 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
   const double vec = double(nsys) * 2.0 * (2.0 * ev * h->nb);
   const double desc = 16.0 * h->nslices;
   *bytes_per_launch = val + idx + vec + desc;
+  return 0;
+}
+
+int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node) {
+  if (!h) return MTCOCR_ERR_ARG;
+  auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
+  if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
+  h->set_caps(cap_spmv, cap_vec, cap_node);
+  drop_graph(h);  // the captured iterations hold the old grids
+  return 0;
+}
+
+int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
+                      int* cap_node) {
+  if (!h) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  const int k = mtcocr_handle::sp_index(h->nsys > 0 ? h->sp : 1), q = precision;
+  if (cap_spmv) *cap_spmv = h->cap_spmv[q][k];
+  if (cap_vec) *cap_vec = h->cap_vec[q][k];
+  if (cap_node) *cap_node = h->cap_node[q][k];
+  return 0;
+}
+
+// Launch-shape autotuner. For the batch currently on the handle (its systems, padding and
+// frequencies) it times the three kernel families whose grids are tunable, each at every
+// candidate workgroup cap, on zero-filled work vectors (the kernels' memory traffic does not
+// depend on the values), and keeps the fastest cap of each family:
+//   SpMV            k_spmv        (cap_spmv; also sets the partial count of its reductions)
+//   node pass       k_pc_node     (cap_node)
+//   edge streams    k_pc_edge_iter, representing k_xr_mrs1 and k_mrs2 (cap_vec)
+// Only work buffers are written; the resident right-hand sides and solutions are untouched.
+int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
+                    int* cap_node, double* spmv_ms) {
+  if (!h || reps < 1) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  const int sp = h->sp;
+  int rc = alloc_work(h, precision, sp);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, sp, 1e-8);
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int ki = mtcocr_handle::sp_index(sp), q = precision;
+  int best[3] = {h->cap_spmv[q][ki], h->cap_node[q][ki], h->cap_vec[q][ki]};
+  double best_ms[3] = {1e30, 1e30, 1e30};
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    auto run = [&](auto tag) -> int {
+      using T = decltype(tag);
+      using T2 = typename Vec2<T>::type;
+      {
+        Engine<T, SP> E(h);
+        const long long nv = (long long)E.nb * SP, nm = (long long)std::max(E.m, 1) * SP;
+        const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
+        const int gm = (int)std::min<long long>(2048, (nm + 255) / 256);
+        for (T2* v : {E.s, E.z, E.a, E.p, E.w}) k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+        k_fill<T2><<<gm, 256, 0, st>>>(E.help, nm);
+        E.setup_diagonals(st);
+        HIPC(hipGetLastError());
+      }
+      for (int fam = 0; fam < 3; ++fam) {
+        int last_grid = -1;
+        for (int cap : kCands) {
+          if (fam == 0) h->cap_spmv[q][ki] = cap;
+          if (fam == 1) h->cap_node[q][ki] = cap;
+          if (fam == 2) h->cap_vec[q][ki] = cap;
+          Engine<T, SP> E(h);
+          const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : E.grid_e;
+          if (grid == last_grid) continue;  // the cap no longer limits this grid
+          last_grid = grid;
+          auto launch = [&]() {
+            if (fam == 0)
+              k_spmv<T, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
+                                                            nullptr, nullptr, E.sc, E.nrun,
+                                                            E.part);
+            else if (fam == 1)
+              k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
+                                                                  E.sc, E.nrun);
+            else
+              k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
+                                                                 E.da, E.help, E.sc, E.nrun,
+                                                                 E.part);
+          };
+          launch();
+          HIPC(hipGetLastError());
+          HIPC(hipEventRecord(h->ev0, st));
+          for (int i = 0; i < reps; ++i) launch();
+          HIPC(hipEventRecord(h->ev1, st));
+          HIPC(hipGetLastError());
+          HIPC(hipEventSynchronize(h->ev1));
+          float ms = 0.f;
+          HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+          const double avg = double(ms) / reps;
+          if (avg < best_ms[fam]) {
+            best_ms[fam] = avg;
+            best[fam] = cap;
+          }
+        }
+        if (fam == 0) h->cap_spmv[q][ki] = best[0];
+        if (fam == 1) h->cap_node[q][ki] = best[1];
+        if (fam == 2) h->cap_vec[q][ki] = best[2];
+      }
+      return 0;
+    };
+    if (precision == MTCOCR_PREC_FP64) return run(double());
+    return run(float());
+  });
+  drop_graph(h);
+  if (rc) return rc;
+  if (cap_spmv) *cap_spmv = h->cap_spmv[q][ki];
+  if (cap_node) *cap_node = h->cap_node[q][ki];
+  if (cap_vec) *cap_vec = h->cap_vec[q][ki];
+  if (spmv_ms) *spmv_ms = best_ms[0];
   return 0;
 }
 
@@ -1663,7 +1807,7 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
       }
       HIPC(hipMemcpyAsync(E.r, xin.data(), xin.size() * sizeof(T2), hipMemcpyHostToDevice, st));
       if (op == 0) {
-        k_spmv<T, SP, 0><<<E.grid_e, kBlock, 0, st>>>(E.A, E.omega, E.r, E.z, nullptr, nullptr,
+        k_spmv<T, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.r, E.z, nullptr, nullptr,
                                                       nullptr, E.sc, E.nrun, E.part);
       } else {
         E.setup_diagonals(st);

@@ -210,3 +210,46 @@ def test_dropin_batch_executable(O, M, c1_model, c1_mesh, tmp_path):
         s = M.read_block_system(d)
         xo, so, _ = O.pcocr(s, c1_mesh, eps=1e-8)
         assert rel(np.fromfile(d / "v3.dat", np.float64), xo) <= 1e-5
+
+
+def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
+    """Every launch shape gives the same solution to rounding (the caps regroup only the dot
+    products) and a deterministic one for a fixed shape; the autotuner keeps a candidate cap."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("launch shapes + autotune: 3 systems")
+    cases = [(0.1, 1), (0.1, 0), (10.0, 1)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    cands = {512, 1024, 2048, 4096, 8192, 16384}
+    with Solver(0) as B:
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        assert B.get_launch() == {"cap_spmv": 8192, "cap_vec": 2048, "cap_node": 8192}
+        x0, s0 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        for caps in [(8, 8, 8), (1024, 512, 16384), (16384, 16384, 1024)]:
+            B.set_launch(*caps)
+            x1, s1 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+            x2, s2 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+            say(f"caps {caps}: iters {[q['iter'] for q in s1]} vs {[q['iter'] for q in s0]}")
+            assert np.array_equal(x1, x2)
+            for k in range(len(cases)):
+                assert s1[k]["req"] == 0
+                assert abs(s1[k]["iter"] - s0[k]["iter"]) <= 2 + 0.02 * s0[k]["iter"]
+                assert rel(x1[k], x0[k]) <= 1e-5
+        with pytest.raises(RuntimeError):
+            B.set_launch(12, 0, 0)  # not a multiple of the XCD count
+        B.set_rhs(prs, om)
+        for prec in ("fp64", "mixed"):
+            t = B.autotune(prec, reps=2)
+            say(f"autotune {prec}: {t}")
+            assert {t["cap_spmv"], t["cap_vec"], t["cap_node"]} <= cands and t["spmv_ms"] > 0
+            assert B.get_launch(prec) == {k: t[k] for k in ("cap_spmv", "cap_vec", "cap_node")}
+        # the resident right-hand sides survive the tuning
+        rc, st = B.solve_resident(1e-8, 10000, "fp64")
+        xr = B.get_solution()
+        for k in range(len(cases)):
+            assert st[k]["req"] == 0 and rel(xr[k], x0[k]) <= 1e-5
