@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="launches per candidate shape in the autotuner")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling mode: autotune, then only the batched SpMV (--spmv-reps "
+                         "launches) and the 1 GiB copy calibration kernel; prints one line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +139,17 @@ def main():
     S.set_rhs(prs[:1], om[:1])
     tune["single_fp64"] = S.autotune("fp64", args.tune_reps)
     log(f"autotuned launch shapes: {tune}")
+
+    if args.kernels_only:
+        S.set_rhs(prs, om)
+        ms, by = S.bench_spmv("fp64", args.spmv_reps)
+        cms = S.bench_copy(1 << 30, args.spmv_reps)
+        print(json.dumps({"spmv_fp64_batched_ms": ms, "spmv_algorithmic_bytes": by,
+                          "spmv_gbs": by / (ms * 1e-3) / 1e9, "copy_bytes": 1 << 30,
+                          "copy_ms": cms, "copy_gbs": 2 * (1 << 30) / (cms * 1e-3) / 1e9,
+                          "launch": tune}), flush=True)
+        S.close()
+        return
 
     # ---- headline: FP64, all systems batched --------------------------------------------
     for w in range(args.warmup):

@@ -15,7 +15,9 @@
  *   mtcocr_get_solution       no single counterpart: they batch the reference's per-(f, pol)
  *                             COCR_FP processes (Code/CalcStarter/main.cpp:291-325) over one
  *                             shared sparsity, A_s = K + i*omega_s*S.
- *   mtcocr_bench_spmv         no counterpart (measurement hook for the SpMV kernel).
+ *   mtcocr_bench_spmv, mtcocr_bench_copy
+ *                             no counterpart (measurement hooks: the SpMV kernel, and a plain
+ *                             copy kernel that calibrates the HBM byte counters).
  *   mtcocr_set_launch, mtcocr_get_launch, mtcocr_autotune
  *                             no counterpart (launch shapes of the GPU kernels; the reference
  *                             is a CPU OpenMP solver with no tunable launch).
@@ -152,6 +154,11 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
  * at its chosen cap. Writes only work buffers: resident right-hand sides and solutions stay. */
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, double* spmv_ms);
+
+/* Calibration hook: average time (ms) of a plain device-to-device copy kernel (k_copy, 16-byte
+ * elements) over nbytes (a multiple of 16), `reps` launches after one warm-up. Its HBM byte
+ * counters (read + write of nbytes each) calibrate the counters of the solver's kernels. */
+int mtcocr_bench_copy(mtcocr_handle* h, int64_t nbytes, int reps, double* avg_ms);
 
 /* Kernel-level hook for parity checks: y = op(x) on the current matrix, op 0 = SpMV (A x),
  * op 1 = folded preconditioner (D_A^-1 x + G^T D_N^-1 G x), in precision 0 (FP64) or 1 (FP32

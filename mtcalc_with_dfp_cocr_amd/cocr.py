@@ -127,6 +127,13 @@ class Solver:
                                         C.byref(ms), C.byref(by)), "mtcocr_bench_spmv")
         return ms.value, by.value
 
+    def bench_copy(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of the plain copy kernel over nbytes (HBM counter calibration)."""
+        ms = C.c_double()
+        check(self._L.mtcocr_bench_copy(self._h, int(nbytes), int(reps), C.byref(ms)),
+              "mtcocr_bench_copy")
+        return ms.value
+
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0) -> None:
         """Workgroup caps of the SpMV / edge-streaming / node kernels (0 keeps a value)."""
         check(self._L.mtcocr_set_launch(self._h, int(cap_spmv), int(cap_vec), int(cap_node)),

@@ -193,6 +193,14 @@ __device__ inline T2 caxpy(T2 x, T2 y, double ar, double ai) {
 // ------------------------------------------------------------------------------------------
 // kernels
 
+// plain streaming copy (16-byte elements): the calibration kernel for the HBM byte counters
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
+                                              double2* __restrict__ dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = i; k < n; k += stride) dst[k] = src[k];
+}
+
 template <class T2>
 __global__ void k_fill(T2* v, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1649,6 +1657,28 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
   const double vec = double(nsys) * 2.0 * (2.0 * ev * h->nb);
   const double desc = 16.0 * h->nslices;
   *bytes_per_launch = val + idx + vec + desc;
+  return 0;
+}
+
+int mtcocr_bench_copy(mtcocr_handle* h, int64_t nbytes, int reps, double* avg_ms) {
+  if (!h || nbytes < 16 || nbytes % 16 || reps < 1 || !avg_ms) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  DevBuf src, dst;
+  int rc;
+  if ((rc = ensure(h, src, (size_t)nbytes)) || (rc = ensure(h, dst, (size_t)nbytes))) return rc;
+  hipStream_t st = h->stream;
+  const long long n = nbytes / 16;
+  k_fill<double2><<<2048, 256, 0, st>>>(src.as<double2>(), n);
+  k_copy<<<8192, 256, 0, st>>>(src.as<double2>(), dst.as<double2>(), n);
+  HIPC(hipEventRecord(h->ev0, st));
+  for (int i = 0; i < reps; ++i)
+    k_copy<<<8192, 256, 0, st>>>(src.as<double2>(), dst.as<double2>(), n);
+  HIPC(hipEventRecord(h->ev1, st));
+  HIPC(hipGetLastError());
+  HIPC(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *avg_ms = double(ms) / reps;
   return 0;
 }
 
