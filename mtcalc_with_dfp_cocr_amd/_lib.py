@@ -77,11 +77,12 @@ SIGNATURES = {
                                           C.c_void_p, C.c_void_p, i32p,
                                           C.POINTER(MtcocrStats)]),
     "mtcocr_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
-    "mtcocr_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mtcocr_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mtcocr_get_launch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
-                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
     "mtcocr_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
-                                  C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),
     "mtcocr_bench_spmv": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),

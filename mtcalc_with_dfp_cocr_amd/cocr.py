@@ -134,25 +134,32 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0) -> None:
-        """Workgroup caps of the SpMV / edge-streaming / node kernels (0 keeps a value)."""
-        check(self._L.mtcocr_set_launch(self._h, int(cap_spmv), int(cap_vec), int(cap_node)),
-              "mtcocr_set_launch")
+    LAYOUTS = {"ell": 0, "rowpack": 1}
+
+    def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
+                   spmv_layout: str | None = None) -> None:
+        """Workgroup caps of the SpMV / edge-streaming / node kernels (0 keeps a value) and the
+        SpMV layout ("ell" or "rowpack"; None keeps it)."""
+        lay = -1 if spmv_layout is None else self.LAYOUTS[spmv_layout]
+        check(self._L.mtcocr_set_launch(self._h, int(cap_spmv), int(cap_vec), int(cap_node),
+                                        lay), "mtcocr_set_launch")
 
     def get_launch(self, precision: str = "fp64") -> dict:
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(self._L.mtcocr_get_launch(self._h, PRECISION[precision], C.byref(a), C.byref(b),
-                                        C.byref(c)), "mtcocr_get_launch")
-        return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value}
+                                        C.byref(c), C.byref(d)), "mtcocr_get_launch")
+        return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
+                "spmv_layout": ("ell", "rowpack")[d.value]}
 
     def autotune(self, precision: str = "fp64", reps: int = 10) -> dict:
-        """Time the tunable kernels of the current batch at each candidate cap and keep the
+        """Time the tunable kernels of the current batch at each candidate shape and keep the
         fastest (needs a batch on the handle: set_rhs or a previous solve_batch)."""
-        a, b, c, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        a, b, c, d, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(self._L.mtcocr_autotune(self._h, PRECISION[precision], int(reps), C.byref(a),
-                                      C.byref(b), C.byref(c), C.byref(ms)), "mtcocr_autotune")
+                                      C.byref(b), C.byref(c), C.byref(d), C.byref(ms)),
+              "mtcocr_autotune")
         return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
-                "spmv_ms": ms.value}
+                "spmv_layout": ("ell", "rowpack")[d.value], "spmv_ms": ms.value}
 
     def apply(self, op: str, x, precision: str = "fp64"):
         """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""

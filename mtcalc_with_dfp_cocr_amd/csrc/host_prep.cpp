@@ -90,6 +90,22 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
     out.offc[s + 1] = int(oc);
     out.offr[s + 1] = int(orr);
   }
+  if (out.nnz_complex + out.nnz_real > 2147483000LL) return MTCOCR_ERR_ARG;
+  // row pack: the diagonal (always stored, complex or real) is left out
+  out.rp.assign(nb + 1, 0);
+  out.rb.assign(nb + 1, 0);
+  std::vector<char> diag_cplx(nb, 0);
+  for (int i = 0; i < nb; ++i)
+    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (ent[k].col == i) diag_cplx[i] = ent[k].cplx;
+  for (int i = 0; i < nb; ++i) {
+    const int offc = rc[i] - (diag_cplx[i] ? 1 : 0), offr = rr[i] - (diag_cplx[i] ? 0 : 1);
+    out.rp[i + 1] = out.rp[i] + offc + offr;
+    out.rb[i + 1] = out.rb[i] + offc;
+  }
+  out.pcol.assign(out.rp[nb], 0);
+  out.pa.assign(out.rp[nb], 0.0);
+  out.pb.assign(out.rb[nb], 0.0);
   const int64_t totc = out.offc[ns], totr = out.offr[ns];
   out.colc.assign(totc, 0);
   out.ac.assign(totc, 0.0);
@@ -116,6 +132,16 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
         ++kr;
       }
     }
+    // row pack: complex off-diagonals, then real off-diagonals (each ascending)
+    int64_t q = out.rp[i], qb = out.rb[i];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const Entry& e = ent[k];
+        if (e.col == i || (e.a == 0.0 && e.b == 0.0) || e.cplx != (pass == 0)) continue;
+        out.pcol[q] = e.col;
+        out.pa[q++] = e.a;
+        if (pass == 0) out.pb[qb++] = e.b;
+      }
     // padding slots point at the row itself with zero value
     for (; kc < out.nc[s]; ++kc) out.colc[out.offc[s] + 64LL * kc + lane] = i;
     for (; kr < out.nr[s]; ++kr) out.colr[out.offr[s] + 64LL * kr + lane] = i;

@@ -24,6 +24,12 @@ struct SlicedEll {
   std::vector<double> ac, bc, ar;
   std::vector<double> diag_a, diag_b;   // diagonal as a + i*omega*b
   int64_t nnz_complex = 0, nnz_real = 0;  // stored (unpadded) entries
+  // Row pack: the off-diagonal entries row-contiguous (no padding) for the row-group SpMV,
+  // where the lanes of one row load consecutive entries together; the diagonal is diag_a/b.
+  // Row i's entries are [rp[i], rp[i+1]) of pcol/pa: first its complex entries (ascending
+  // columns), whose b values are [rb[i], rb[i+1]) of pb, then its real ones (ascending).
+  std::vector<int> rp, rb, pcol;
+  std::vector<double> pa, pb;
 };
 
 struct GradientEll {

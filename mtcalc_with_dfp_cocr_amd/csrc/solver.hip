@@ -71,6 +71,14 @@ struct EllView {
   int nb;
 };
 
+// row pack: row-contiguous off-diagonal entries, complex ones first (see host_prep.h), and the
+// diagonal as da + i*omega*db; FP64 values
+struct RowView {
+  const int *rp, *rb, *col;
+  const double *a, *b, *da, *db;
+  int nb;
+};
+
 template <class T>
 struct GView {
   const int *gcol, *gtcol;
@@ -300,6 +308,40 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
   }
 }
 
+// Output of one (row, system) of the SpMV kernels: y = A x with the partial (y, x) (MODE 0), or
+// r = b - A x with the partials ||b||^2, ||r||^2 and the MRS initialisation (MODE 1, 2).
+template <class T, int MODE>
+__device__ inline void spmv_store(long long t, double yr, double yi,
+                                  const typename Vec2<T>::type* __restrict__ x,
+                                  typename Vec2<T>::type* __restrict__ y,
+                                  const typename Vec2<T>::type* __restrict__ b,
+                                  typename Vec2<T>::type* __restrict__ ym,
+                                  typename Vec2<T>::type* __restrict__ sm, double (&v)[2]) {
+  using T2 = typename Vec2<T>::type;
+  if (MODE == 0) {
+    T2 o;
+    o.x = T(yr);
+    o.y = T(yi);
+    y[t] = o;
+    const T2 xs = x[t];
+    // the dot uses the stored (rounded) product, as the reference's ScalCmplx(z, s) does
+    v[0] += double(o.x) * double(xs.x) - double(o.y) * double(xs.y);
+    v[1] += double(o.x) * double(xs.y) + double(o.y) * double(xs.x);
+  } else {
+    const T2 bv = b[t];
+    T2 o;
+    o.x = T(double(bv.x) - yr);
+    o.y = T(double(bv.y) - yi);
+    y[t] = o;
+    v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
+    v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
+    if (ym) {
+      ym[t] = x[t];
+      sm[t] = o;
+    }
+  }
+}
+
 // y = A x (MODE 0, partial (y, x) unconjugated) or r = b - A x (MODE 1, partials ||b||^2 and
 // ||r||^2, plus the MRS initialisation ym = x, sm = r of MRS.cpp:91-97). MODE 2 is MODE 1 gated
 // by the outer (refinement) status instead of the COCR status.
@@ -357,28 +399,125 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<double> A, const double
         yi += a * double(xv.y);
       }
     }
-    if (MODE == 0) {
-      T2 o;
-      o.x = T(yr);
-      o.y = T(yi);
-      y[t] = o;
-      const T2 xs = x[t];
-      // the dot uses the stored (rounded) product, as the reference's ScalCmplx(z, s) does
-      v[0] += double(o.x) * double(xs.x) - double(o.y) * double(xs.y);
-      v[1] += double(o.x) * double(xs.y) + double(o.y) * double(xs.x);
-    } else {
-      const T2 bv = b[t];
-      T2 o;
-      o.x = T(double(bv.x) - yr);
-      o.y = T(double(bv.y) - yi);
-      y[t] = o;
-      v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
-      v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
-      if (ym) {
-        ym[t] = x[t];
-        sm[t] = o;
+    spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// Broadcast of lane J of each SP-lane row group to the whole group: DPP quad_perm (SP 4),
+// DPP row_newbcast within 16-lane rows (SP 8, 16), v_readlane (SP 32, 64). Every lane of a
+// group is active at the call (groups are whole rows).
+template <int SP, int J>
+__device__ inline int grp_bcast(int v) {
+  if constexpr (SP == 4) {
+    return __builtin_amdgcn_update_dpp(0, v, J | (J << 2) | (J << 4) | (J << 6), 0xF, 0xF, false);
+  } else if constexpr (SP == 8) {
+    const int lo = __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, v, 0x150 + 8 + J, 0xF, 0xF, false);
+    return (threadIdx.x & 8) ? hi : lo;
+  } else if constexpr (SP == 16) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+  } else if constexpr (SP == 32) {
+    const int lo = __builtin_amdgcn_readlane(v, J);
+    const int hi = __builtin_amdgcn_readlane(v, 32 + J);
+    return (threadIdx.x & 32) ? hi : lo;
+  } else {
+    static_assert(SP == 64, "row groups of 4..64 lanes");
+    return __builtin_amdgcn_readlane(v, J);
+  }
+}
+
+template <int SP, int J>
+__device__ inline double grp_bcast_d(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = grp_bcast<SP, J>(int(u & 0xffffffffLL));
+  const int hi = grp_bcast<SP, J>(int(u >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int J, int N, class F>
+__device__ inline void static_for(F&& f) {
+  if constexpr (J < N) {
+    f(std::integral_constant<int, J>{});
+    static_for<J + 1, N>(f);
+  }
+}
+
+// One chunk of up to SP entries of a row: lane l of the group holds entry k0 + l (b = 0 for a
+// real entry); the entries are broadcast one at a time and every lane gathers its system's x.
+// Full chunks run branch-free so that all SP gathers are in flight together; lanes of retired
+// systems read x[s] (row 0, cache-resident) instead of branching around the gather.
+template <class T, int SP>
+__device__ inline void rg_chunk(int cnt, int c, double a, double bb, double w, bool run,
+                                const typename Vec2<T>::type* __restrict__ x, int s, double& yr,
+                                double& yi) {
+  using T2 = typename Vec2<T>::type;
+  auto step = [&](auto J) {
+    const int cj = grp_bcast<SP, J>(c);
+    const double aj = grp_bcast_d<SP, J>(a);
+    const double bj = grp_bcast_d<SP, J>(bb) * w;
+    const size_t xi = run ? (size_t)cj * SP + s : (size_t)s;
+    const T2 xv = x[xi];
+    const double xr = double(xv.x), xm = double(xv.y);
+    yr += aj * xr - bj * xm;
+    yi += aj * xm + bj * xr;
+  };
+  if (cnt >= SP) {
+    static_for<0, SP>(step);
+  } else {
+    static_for<0, SP>([&](auto J) {
+      if (J < cnt) step(J);
+    });
+  }
+}
+
+// Row-group SpMV over the row pack: SP lanes per row (one per system), the row's entries
+// loaded SP at a time with one coalesced load per lane and broadcast inside the group, so the
+// matrix costs one load per entry per group instead of one per entry per lane. Same modes,
+// outputs and partials as k_spmv; the diagonal is applied first.
+template <class T, int SP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* omega,
+                                                    const typename Vec2<T>::type* __restrict__ x,
+                                                    typename Vec2<T>::type* __restrict__ y,
+                                                    const typename Vec2<T>::type* __restrict__ b,
+                                                    typename Vec2<T>::type* __restrict__ ym,
+                                                    typename Vec2<T>::type* __restrict__ sm,
+                                                    const SysScalars* __restrict__ sc,
+                                                    const int* __restrict__ nrun,
+                                                    double* partials) {
+  static_assert(SP >= 4 && kBlock % SP == 0, "row groups of 4..64 lanes");
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const long long N = (long long)A.nb * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  double v[2] = {0.0, 0.0};
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;  // N is a multiple of SP: whole groups drop out together
+    const int s = int(t % SP);
+    const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
+    const int row = int(t / SP);
+    const double w = omega[s];
+    const T2 xs = x[t];
+    const double dA = A.da[row], dB = A.db[row] * w;
+    double yr = dA * double(xs.x) - dB * double(xs.y);
+    double yi = dA * double(xs.y) + dB * double(xs.x);
+    const int beg = A.rp[row], end = A.rp[row + 1];
+    const int bbeg = A.rb[row], ncx = A.rb[row + 1] - bbeg;
+    for (int k0 = beg; k0 < end; k0 += SP) {
+      const int l = k0 - beg + s;
+      int c = 0;
+      double a = 0.0, bb = 0.0;
+      if (k0 + s < end) {
+        c = A.col[k0 + s];
+        a = A.a[k0 + s];
+        if (l < ncx) bb = A.b[bbeg + l];
       }
+      rg_chunk<T, SP>(end - k0, c, a, bb, w, run, x, s, yr, yi);
     }
+    if (!run) continue;
+    spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -859,6 +998,7 @@ struct mtcocr_handle {
   int64_t nnz_c = 0, nnz_r = 0;
   DevBuf offc, nc, offr, nr, colc, colr;
   DevBuf ac64, bc64, ar64, da64, db64;
+  DevBuf rp, rb, pcol, pa, pb;  // row pack of the off-diagonal entries
   // gradient
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0;
@@ -876,24 +1016,32 @@ struct mtcocr_handle {
   // launch shapes (workgroup caps of the grid-stride kernels) per vector precision (FP64,
   // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
   int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
+  // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg,
+  // batch widths 4..64)
+  int spmv_layout[2][7];
   static int sp_index(int sp) {
     int k = 0;
     while ((1 << k) < sp && k < 6) ++k;
     return k;
   }
-  void set_caps(int cs, int cv, int cn) {
+  void set_caps(int cs, int cv, int cn, int layout) {
     for (int p = 0; p < 2; ++p)
       for (int k = 0; k < 7; ++k) {
         if (cs) cap_spmv[p][k] = cs;
         if (cv) cap_vec[p][k] = cv;
         if (cn) cap_node[p][k] = cn;
+        if (layout >= 0) spmv_layout[p][k] = (k >= 2) ? layout : 0;
       }
   }
-  mtcocr_handle() { set_caps(8192, 2048, 8192); }
+  mtcocr_handle() {
+    set_caps(8192, 2048, 8192, 0);
+    for (int p = 0; p < 2; ++p)
+      for (int k = 4; k < 7; ++k) spmv_layout[p][k] = 1;  // row pack from 16 systems up
+  }
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
-                           &db64, &gcol, &gtcol, &gval64, &gtval64,
+                           &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vda, &vdn,
                            &vx, &vb};
@@ -962,6 +1110,38 @@ EllView<T> ell_view(mtcocr_handle* h) {
   return v;
 }
 
+RowView row_view(mtcocr_handle* h) {
+  RowView v;
+  v.rp = h->rp.as<int>();
+  v.rb = h->rb.as<int>();
+  v.col = h->pcol.as<int>();
+  v.a = h->pa.as<double>();
+  v.b = h->pb.as<double>();
+  v.da = h->da64.as<double>();
+  v.db = h->db64.as<double>();
+  v.nb = h->nb;
+  return v;
+}
+
+// SpMV launch in the layout chosen for this batch width (vector type TV, which the outer FP64
+// residual of the mixed solver uses with an FP32 engine).
+template <class TV, int SP, int MODE>
+void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const double* omega,
+                 const typename Vec2<TV>::type* x, typename Vec2<TV>::type* y,
+                 const typename Vec2<TV>::type* b, typename Vec2<TV>::type* ym,
+                 typename Vec2<TV>::type* sm, const SysScalars* sc, const int* nrun,
+                 double* part) {
+  if constexpr (SP >= 4) {
+    if (layout == 1) {
+      k_spmv_rg<TV, SP, MODE><<<grid, kBlock, 0, st>>>(row_view(h), omega, x, y, b, ym, sm, sc,
+                                                       nrun, part);
+      return;
+    }
+  }
+  k_spmv<TV, SP, MODE><<<grid, kBlock, 0, st>>>(ell_view<double>(h), omega, x, y, b, ym, sm, sc,
+                                                nrun, part);
+}
+
 template <class T>
 GView<T> g_view(mtcocr_handle* h) {
   GView<T> g;
@@ -993,7 +1173,7 @@ struct Engine {
   int* nrun;
   double* part;
   const double* omega;
-  int nb, m, grid_s, grid_e, grid_n;
+  int nb, m, grid_s, grid_e, grid_n, layout;
 
   explicit Engine(mtcocr_handle* hh) : h(hh) {
     A = ell_view<double>(h);
@@ -1021,6 +1201,14 @@ struct Engine {
     grid_s = grid_for((long long)nb * SP, h->cap_spmv[q][k]);
     grid_e = grid_for((long long)nb * SP, h->cap_vec[q][k]);
     grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
+    layout = h->spmv_layout[q][k];
+  }
+
+  template <int MODE, class TV>
+  void spmv(hipStream_t st, const TV* xin, TV* yout, const TV* bin, TV* ymo, TV* smo) {
+    using E = decltype(xin->x);
+    launch_spmv<E, SP, MODE>(h, layout, grid_s, st, omega, xin, yout, bin, ymo, smo, sc, nrun,
+                             part);
   }
 
   void setup_diagonals(hipStream_t st) {
@@ -1032,7 +1220,7 @@ struct Engine {
 
   // stages 1-4 of the request order: r = b - A x, X0 test, s = M^-1 r, p = a = w = 0, z = A s
   void prologue(hipStream_t st, int maxiter) {
-    k_spmv<T, SP, 1><<<grid_s, kBlock, 0, st>>>(A, omega, x, r, b, ym, sm, sc, nrun, part);
+    spmv<1>(st, x, r, b, ym, sm);
     s_prologue<SP><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun);
     k_pc_node<T, SP, false><<<grid_n, kBlock, 0, st>>>(G, r, nullptr, dn, help, sc, nrun);
     k_pc_edge_plain<T, SP><<<grid_e, kBlock, 0, st>>>(G, r, da, help, s, sc, nrun);
@@ -1041,8 +1229,7 @@ struct Engine {
     k_fill<T2><<<gf, 256, 0, st>>>(p, nv);
     k_fill<T2><<<gf, 256, 0, st>>>(a, nv);
     k_fill<T2><<<gf, 256, 0, st>>>(w, nv);
-    k_spmv<T, SP, 0><<<grid_s, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
-                                                nrun, part);
+    spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
   }
 
@@ -1055,8 +1242,7 @@ struct Engine {
     s_mrs<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc);
     k_mrs2<T, SP><<<grid_e, kBlock, 0, st>>>(x, r, ym, sm, s, w, sc, nrun, part, nb);
     s_stop<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
-    k_spmv<T, SP, 0><<<grid_s, kBlock, 0, st>>>(A, omega, s, z, nullptr, nullptr, nullptr, sc,
-                                                nrun, part);
+    spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
   }
 };
@@ -1189,8 +1375,7 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   for (int s = 0; s < SP; ++s) tmp[s].status = (s < h->nsys) ? ST_RUNNING : ST_INACTIVE;
   HIPC(hipMemcpyAsync(E.sc, tmp.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  k_spmv<double, SP, 1><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.x, h->r64.as<double2>(), E.b,
-                                                     nullptr, nullptr, E.sc, E.nrun, E.part);
+  E.template spmv<1>(st, E.x, h->r64.as<double2>(), E.b, (double2*)nullptr, (double2*)nullptr);
   HIPC(hipGetLastError());
   std::vector<double> partials((size_t)E.grid_s * SP * 2);
   HIPC(hipMemcpyAsync(partials.data(), E.part, partials.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1228,7 +1413,6 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
   float ms = 0.f;
   {
     Engine<float, SP> E(h);
-    EllView<double> A64 = ell_view<double>(h);
     set_eps_status(h, hs, h->nsys, SP, eps);
     for (int s = 0; s < SP; ++s) hs[s].stall_window = 200;
     HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
@@ -1248,9 +1432,9 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
         return MTCOCR_ERR_HIP;
       }
       // FP64 true residual r64 = b - A x64 on the systems still refining
-      k_spmv<double, SP, 2><<<E.grid_s, kBlock, 0, st>>>(A64, E.omega, h->x64.as<double2>(),
-                                                         h->r64.as<double2>(), h->b64.as<double2>(),
-                                                         nullptr, nullptr, E.sc, nrun_outer, E.part);
+      launch_spmv<double, SP, 2>(h, E.layout, E.grid_s, st, E.omega, h->x64.as<double2>(),
+                                 h->r64.as<double2>(), h->b64.as<double2>(), nullptr, nullptr,
+                                 E.sc, nrun_outer, E.part);
       s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_s, E.sc, nrun_outer, E.nrun, eps,
                                                 inner_floor, pass == 0 ? 1 : 0, max_outer);
       HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1397,7 +1581,9 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
       (rc = upload(h, h->colc, E.colc, st)) || (rc = upload(h, h->colr, E.colr, st)) ||
       (rc = upload(h, h->ac64, E.ac, st)) || (rc = upload(h, h->bc64, E.bc, st)) ||
       (rc = upload(h, h->ar64, E.ar, st)) || (rc = upload(h, h->da64, E.diag_a, st)) ||
-      (rc = upload(h, h->db64, E.diag_b, st)))
+      (rc = upload(h, h->db64, E.diag_b, st)) || (rc = upload(h, h->rp, E.rp, st)) ||
+      (rc = upload(h, h->rb, E.rb, st)) || (rc = upload(h, h->pcol, E.pcol, st)) ||
+      (rc = upload(h, h->pa, E.pa, st)) || (rc = upload(h, h->pb, E.pb, st)))
     return rc;
   HIPC(hipStreamSynchronize(st));
   h->have_matrix = true;
@@ -1623,22 +1809,22 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
     if (precision == MTCOCR_PREC_FP64) {
       Engine<double, SP> E(h);
       k_fill<double2><<<gf, 256, 0, st>>>(E.s, nv);
-      k_spmv<double, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
-                                                         nullptr, E.sc, E.nrun, E.part);
+      E.template spmv<0>(st, E.s, E.z, (const double2*)nullptr, (double2*)nullptr,
+                         (double2*)nullptr);
       HIPC(hipEventRecord(h->ev0, st));
       for (int i = 0; i < reps; ++i)
-        k_spmv<double, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
-                                                           nullptr, nullptr, E.sc, E.nrun, E.part);
+        E.template spmv<0>(st, E.s, E.z, (const double2*)nullptr, (double2*)nullptr,
+                           (double2*)nullptr);
       HIPC(hipEventRecord(h->ev1, st));
     } else {
       Engine<float, SP> E(h);
       k_fill<float2><<<gf, 256, 0, st>>>(E.s, nv);
-      k_spmv<float, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr, nullptr,
-                                                        nullptr, E.sc, E.nrun, E.part);
+      E.template spmv<0>(st, E.s, E.z, (const float2*)nullptr, (float2*)nullptr,
+                         (float2*)nullptr);
       HIPC(hipEventRecord(h->ev0, st));
       for (int i = 0; i < reps; ++i)
-        k_spmv<float, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
-                                                          nullptr, nullptr, E.sc, E.nrun, E.part);
+        E.template spmv<0>(st, E.s, E.z, (const float2*)nullptr, (float2*)nullptr,
+                           (float2*)nullptr);
       HIPC(hipEventRecord(h->ev1, st));
     }
     HIPC(hipGetLastError());
@@ -1682,36 +1868,40 @@ int mtcocr_bench_copy(mtcocr_handle* h, int64_t nbytes, int reps, double* avg_ms
   return 0;
 }
 
-int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node) {
+int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
+                      int spmv_layout) {
   if (!h) return MTCOCR_ERR_ARG;
   auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
   if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
-  h->set_caps(cap_spmv, cap_vec, cap_node);
+  if (spmv_layout < -1 || spmv_layout > 1) return MTCOCR_ERR_ARG;
+  h->set_caps(cap_spmv, cap_vec, cap_node, spmv_layout);
   drop_graph(h);  // the captured iterations hold the old grids
   return 0;
 }
 
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
-                      int* cap_node) {
+                      int* cap_node, int* spmv_layout) {
   if (!h) return MTCOCR_ERR_ARG;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
   const int k = mtcocr_handle::sp_index(h->nsys > 0 ? h->sp : 1), q = precision;
   if (cap_spmv) *cap_spmv = h->cap_spmv[q][k];
   if (cap_vec) *cap_vec = h->cap_vec[q][k];
   if (cap_node) *cap_node = h->cap_node[q][k];
+  if (spmv_layout) *spmv_layout = h->spmv_layout[q][k];
   return 0;
 }
 
 // Launch-shape autotuner. For the batch currently on the handle (its systems, padding and
-// frequencies) it times the three kernel families whose grids are tunable, each at every
-// candidate workgroup cap, on zero-filled work vectors (the kernels' memory traffic does not
-// depend on the values), and keeps the fastest cap of each family:
-//   SpMV            k_spmv        (cap_spmv; also sets the partial count of its reductions)
-//   node pass       k_pc_node     (cap_node)
-//   edge streams    k_pc_edge_iter, representing k_xr_mrs1 and k_mrs2 (cap_vec)
+// frequencies) it times the kernel families whose launch is tunable, each at every candidate
+// workgroup cap, on zero-filled work vectors (the kernels' memory traffic does not depend on
+// the values), and keeps the fastest choice of each family:
+//   SpMV            layout (sliced ELL k_spmv, or row pack k_spmv_rg for 4..64 systems) x cap
+//                   (the cap also sets the partial count of its reductions)
+//   node pass       k_pc_node cap
+//   edge streams    k_pc_edge_iter cap, which stands for k_xr_mrs1 and k_mrs2 as well
 // Only work buffers are written; the resident right-hand sides and solutions are untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
-                    int* cap_node, double* spmv_ms) {
+                    int* cap_node, int* spmv_layout, double* spmv_ms) {
   if (!h || reps < 1) return MTCOCR_ERR_ARG;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
   if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
@@ -1727,7 +1917,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
   static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384};
   const int ki = mtcocr_handle::sp_index(sp), q = precision;
-  int best[3] = {h->cap_spmv[q][ki], h->cap_node[q][ki], h->cap_vec[q][ki]};
+  int* caps[3] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki]};
+  int best[3] = {*caps[0], *caps[1], *caps[2]};
+  int best_layout = h->spmv_layout[q][ki];
   double best_ms[3] = {1e30, 1e30, 1e30};
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
@@ -1745,46 +1937,47 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         HIPC(hipGetLastError());
       }
       for (int fam = 0; fam < 3; ++fam) {
-        int last_grid = -1;
-        for (int cap : kCands) {
-          if (fam == 0) h->cap_spmv[q][ki] = cap;
-          if (fam == 1) h->cap_node[q][ki] = cap;
-          if (fam == 2) h->cap_vec[q][ki] = cap;
-          Engine<T, SP> E(h);
-          const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : E.grid_e;
-          if (grid == last_grid) continue;  // the cap no longer limits this grid
-          last_grid = grid;
-          auto launch = [&]() {
-            if (fam == 0)
-              k_spmv<T, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.s, E.z, nullptr,
-                                                            nullptr, nullptr, E.sc, E.nrun,
-                                                            E.part);
-            else if (fam == 1)
-              k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
-                                                                  E.sc, E.nrun);
-            else
-              k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
-                                                                 E.da, E.help, E.sc, E.nrun,
-                                                                 E.part);
-          };
-          launch();
-          HIPC(hipGetLastError());
-          HIPC(hipEventRecord(h->ev0, st));
-          for (int i = 0; i < reps; ++i) launch();
-          HIPC(hipEventRecord(h->ev1, st));
-          HIPC(hipGetLastError());
-          HIPC(hipEventSynchronize(h->ev1));
-          float ms = 0.f;
-          HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-          const double avg = double(ms) / reps;
-          if (avg < best_ms[fam]) {
-            best_ms[fam] = avg;
-            best[fam] = cap;
+        const int nlayouts = (fam == 0 && SP >= 4) ? 2 : 1;
+        for (int layout = 0; layout < nlayouts; ++layout) {
+          int last_grid = -1;
+          for (int cap : kCands) {
+            *caps[fam] = cap;
+            if (fam == 0) h->spmv_layout[q][ki] = layout;
+            Engine<T, SP> E(h);
+            const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : E.grid_e;
+            if (grid == last_grid) continue;  // the cap no longer limits this grid
+            last_grid = grid;
+            auto launch = [&]() {
+              if (fam == 0)
+                E.template spmv<0>(st, E.s, E.z, (const T2*)nullptr, (T2*)nullptr,
+                                   (T2*)nullptr);
+              else if (fam == 1)
+                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
+                                                                    E.sc, E.nrun);
+              else
+                k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
+                                                                   E.da, E.help, E.sc, E.nrun,
+                                                                   E.part);
+            };
+            launch();
+            HIPC(hipGetLastError());
+            HIPC(hipEventRecord(h->ev0, st));
+            for (int i = 0; i < reps; ++i) launch();
+            HIPC(hipEventRecord(h->ev1, st));
+            HIPC(hipGetLastError());
+            HIPC(hipEventSynchronize(h->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+            const double avg = double(ms) / reps;
+            if (avg < best_ms[fam]) {
+              best_ms[fam] = avg;
+              best[fam] = cap;
+              if (fam == 0) best_layout = layout;
+            }
           }
         }
-        if (fam == 0) h->cap_spmv[q][ki] = best[0];
-        if (fam == 1) h->cap_node[q][ki] = best[1];
-        if (fam == 2) h->cap_vec[q][ki] = best[2];
+        *caps[fam] = best[fam];
+        if (fam == 0) h->spmv_layout[q][ki] = best_layout;
       }
       return 0;
     };
@@ -1796,6 +1989,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   if (cap_spmv) *cap_spmv = h->cap_spmv[q][ki];
   if (cap_node) *cap_node = h->cap_node[q][ki];
   if (cap_vec) *cap_vec = h->cap_vec[q][ki];
+  if (spmv_layout) *spmv_layout = h->spmv_layout[q][ki];
   if (spmv_ms) *spmv_ms = best_ms[0];
   return 0;
 }
@@ -1837,8 +2031,7 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
       }
       HIPC(hipMemcpyAsync(E.r, xin.data(), xin.size() * sizeof(T2), hipMemcpyHostToDevice, st));
       if (op == 0) {
-        k_spmv<T, SP, 0><<<E.grid_s, kBlock, 0, st>>>(E.A, E.omega, E.r, E.z, nullptr, nullptr,
-                                                      nullptr, E.sc, E.nrun, E.part);
+        E.template spmv<0>(st, E.r, E.z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
       } else {
         E.setup_diagonals(st);
         k_pc_node<T, SP, false><<<E.grid_n, kBlock, 0, st>>>(E.G, E.r, nullptr, E.dn, E.help,

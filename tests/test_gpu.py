@@ -59,6 +59,29 @@ def test_spmv_parity(S, O, M, c1_system, prec, tol):
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nsys", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("layout", ["ell", "rowpack"])
+def test_spmv_parity_batch_widths(O, M, c1_system, c1_mesh, nsys, layout):
+    """Both SpMV layouts at every batch width with a row-group kernel (4..64 lanes per row):
+    system 0 of the batch against the oracle, FP64 and FP32 vectors."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say(f"spmv layout {layout}, {nsys} systems")
+    x = np.random.default_rng(13).standard_normal(2 * c1_system.nb)
+    ref = O.spmv(c1_system, x)
+    with Solver(0) as B:
+        B.set_matrix(c1_system)
+        B.set_gradient(c1_mesh)
+        B.set_rhs(np.tile(c1_system.pr, (nsys, 1)))
+        B.set_launch(spmv_layout=layout)
+        assert B.get_launch()["spmv_layout"] == layout
+        for prec, tol in (("fp64", 1e-13), ("mixed", 2e-6)):
+            got = B.apply("spmv", x, prec)
+            assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
 def test_preconditioner_parity(S, O, M, c1_system, c1_mesh, prec, tol):
     say(f"preconditioner {prec}")
@@ -228,9 +251,11 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
     with Solver(0) as B:
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(c1_mesh)
-        assert B.get_launch() == {"cap_spmv": 8192, "cap_vec": 2048, "cap_node": 8192}
+        assert B.get_launch() == {"cap_spmv": 8192, "cap_vec": 2048, "cap_node": 8192,
+                                  "spmv_layout": "ell"}
         x0, s0 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
-        for caps in [(8, 8, 8), (1024, 512, 16384), (16384, 16384, 1024)]:
+        for caps in [(8, 8, 8, "rowpack"), (1024, 512, 16384, "ell"),
+                     (16384, 16384, 1024, "rowpack")]:
             B.set_launch(*caps)
             x1, s1 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
             x2, s2 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
@@ -247,7 +272,8 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
             t = B.autotune(prec, reps=2)
             say(f"autotune {prec}: {t}")
             assert {t["cap_spmv"], t["cap_vec"], t["cap_node"]} <= cands and t["spmv_ms"] > 0
-            assert B.get_launch(prec) == {k: t[k] for k in ("cap_spmv", "cap_vec", "cap_node")}
+            assert B.get_launch(prec) == {k: t[k] for k in ("cap_spmv", "cap_vec", "cap_node",
+                                                            "spmv_layout")}
         # the resident right-hand sides survive the tuning
         rc, st = B.solve_resident(1e-8, 10000, "fp64")
         xr = B.get_solution()
