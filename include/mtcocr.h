@@ -103,8 +103,9 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
 int mtcocr_solve(mtcocr_handle* h, const double* pr, double* x, double eps, int maxiter,
                  int precision, mtcocr_stats* stats);
 
-/* Batched solve of nsys systems A(omega_s) x_s = b_s sharing the matrix set by
- * mtcocr_set_matrix_ks (or, with omega == NULL and nsys == 1, the block2x2 matrix).
+/* Batched solve of nsys (1..64) systems A(omega_s) x_s = b_s sharing the matrix set by
+ * mtcocr_set_matrix_ks, or, with omega == NULL, nsys right-hand sides of the block2x2 matrix
+ * (for example both polarizations of one frequency).
  * pr and x hold nsys consecutive vectors of 2*nb doubles; stats has nsys entries.
  * Systems retire independently. Returns 0 when every system returned MTCOCR_OK, otherwise the
  * first non-zero per-system code. */

@@ -1700,7 +1700,7 @@ int mtcocr_set_rhs(mtcocr_handle* h, int nsys, const double* omega, const double
                    const double* x0) {
   if (!h || !pr) return MTCOCR_ERR_ARG;
   if (!h->have_matrix) return MTCOCR_ERR_STATE;
-  if (!h->ks_form && (nsys != 1 || omega)) return MTCOCR_ERR_ARG;
+  if (!h->ks_form && omega) return MTCOCR_ERR_ARG;  // one matrix, several right-hand sides
   if (h->ks_form && !omega) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
   int rc = set_batch(h, nsys, omega);
