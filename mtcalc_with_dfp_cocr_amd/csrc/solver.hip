@@ -61,7 +61,7 @@ struct SysScalars {
   double scale, prev_rel, b_norm, best_rel;
   int mode, status, iter, total_iter;
   int outer_status, outer, last_improve, stall_window;
-  int fallback, fallback_iter, pad0, pad1;
+  int fallback, fallback_iter, rollback, pad1;
 };
 
 template <class T>
@@ -221,21 +221,22 @@ __global__ void k_fill(T2* v, long long n) {
   }
 }
 
-// D_A^-1 from the FP64 diagonal: 1 / (a + i*omega*b) (FoldedPreconditioner.cpp:507-520)
-template <class T, int SP>
-__global__ void k_diag_a(const double* da, const double* db, const double* omega,
-                         typename Vec2<T>::type* out, int nb) {
+// D_A^-1 = 1 / (a + i*omega*b) from the FP64 matrix diagonal (FoldedPreconditioner.cpp:507-520),
+// evaluated where it is used (16 bytes per row, shared by the batch) instead of being stored
+// as a per-system vector that every application would read.
+struct DiagA {
+  const double *a, *b, *omega;
+};
+
+template <class T>
+__device__ inline typename Vec2<T>::type diag_inv(const DiagA& D, int row, int s) {
 #pragma clang fp contract(off)
-  const long long n = (long long)nb * SP;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const long long row = t / SP;
-    const int s = int(t % SP);
-    double qr, qi;
-    cdiv(1.0, 0.0, da[row], omega[s] * db[row], qr, qi);
-    out[t].x = T(qr);
-    out[t].y = T(qi);
-  }
+  double qr, qi;
+  cdiv(1.0, 0.0, D.a[row], D.omega[s] * D.b[row], qr, qi);
+  typename Vec2<T>::type o;
+  o.x = T(qr);
+  o.y = T(qi);
+  return o;
 }
 
 // value of A(j, c) = a + i*omega*b in row j of the sliced ELL (0 when absent)
@@ -563,10 +564,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
 // Prologue form: out = D_A^-1 v + G^T help (FoldedPreconditioner.cpp:585-589,618-627)
 template <class T, int SP>
 __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
-    GView<T> G, const typename Vec2<T>::type* __restrict__ v,
-    const typename Vec2<T>::type* __restrict__ da, const typename Vec2<T>::type* __restrict__ help,
-    typename Vec2<T>::type* __restrict__ out, const SysScalars* __restrict__ sc,
-    const int* __restrict__ nrun) {
+    GView<T> G, const typename Vec2<T>::type* __restrict__ v, DiagA da,
+    const typename Vec2<T>::type* __restrict__ help, typename Vec2<T>::type* __restrict__ out,
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun) {
   using T2 = typename Vec2<T>::type;
   if (*nrun == 0) return;
   const long long N = (long long)G.n * SP;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
     const int s = int(t % SP);
     if (sc[s].status != ST_RUNNING) continue;
     const int e = int(t / SP);
-    const T2 xv = v[t], d = da[t];
+    const T2 xv = v[t], d = diag_inv<T>(da, e, s);
     T yr = xv.x * d.x - xv.y * d.y;
     T yi = xv.x * d.y + xv.y * d.x;
     for (int k = 0; k < G.wgt; ++k) {
@@ -600,7 +600,7 @@ template <class T, int SP>
 __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     GView<T> G, const typename Vec2<T>::type* __restrict__ z, typename Vec2<T>::type* __restrict__ a,
     const typename Vec2<T>::type* __restrict__ s_vec, typename Vec2<T>::type* __restrict__ p,
-    typename Vec2<T>::type* __restrict__ w, const typename Vec2<T>::type* __restrict__ da,
+    typename Vec2<T>::type* __restrict__ w, DiagA da,
     const typename Vec2<T>::type* __restrict__ help, const SysScalars* __restrict__ sc,
     const int* __restrict__ nrun, double* partials) {
   using T2 = typename Vec2<T>::type;
@@ -618,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     const double br = sc[s].be_re, bi = sc[s].be_im;
     const T2 an = caxpy(z[t], a[t], br, bi);
     const T2 pn = caxpy(s_vec[t], p[t], br, bi);
-    const T2 d = da[t];
+    const T2 d = diag_inv<T>(da, e, s);
     T yr = an.x * d.x - an.y * d.y;
     T yi = an.x * d.y + an.y * d.x;
     for (int k = 0; k < G.wgt; ++k) {
@@ -855,6 +855,15 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
   if (threadIdx.x == 0) count_running(sc, SP, nrun);
 }
 
+// the partial-sum reduction of a scalar kernel alone (the autotuner times kernel + reduction)
+template <int P>
+__global__ __launch_bounds__(kScalarThreads) void s_reduce_probe(const double* part, int nblk,
+                                                                 double* out) {
+  __shared__ double tot[P];
+  reduce_partials<P>(part, nblk, tot);
+  if (threadIdx.x < P) out[threadIdx.x] = tot[threadIdx.x];
+}
+
 // ---- mixed precision: FP64 outer refinement ------------------------------------------------
 
 // after the FP64 true residual: relative residual, convergence / stall, inner set-up
@@ -868,6 +877,7 @@ __global__ __launch_bounds__(kScalarThreads) void s_outer(const double* part, in
   reduce_partials<SP * 2>(part, nblk, tot);
   const int s = threadIdx.x;
   if (s < SP) {
+    sc[s].rollback = 0;
     if (sc[s].outer_status == ST_RUNNING) {
       const double bb = tot[s * 2], rr = tot[s * 2 + 1];
       const double rn = rr > 0.0 ? sqrt(rr) : 0.0;
@@ -887,8 +897,11 @@ __global__ __launch_bounds__(kScalarThreads) void s_outer(const double* part, in
         if (rel < eps) {
           sc[s].outer_status = MTCOCR_OK;
         } else if (!first && rel > 0.1 * sc[s].prev_rel) {
-          // a refinement pass that gains less than 10x hands the system to the FP64 solver
+          // a refinement pass that gains less than 10x hands the system to the FP64 solver;
+          // one that made the residual worse is undone first (k_rollback)
           sc[s].outer_status = MTCOCR_ERR_NOTCONV;
+          sc[s].rollback = rel > sc[s].prev_rel ? 1 : 0;
+          if (sc[s].rollback) sc[s].rel = sc[s].prev_rel;
         } else if (sc[s].outer >= max_outer) {
           sc[s].outer_status = MTCOCR_ERR_NOTCONV;
         } else {
@@ -944,6 +957,20 @@ __global__ void k_from_inner(double2* x64, const float2* y32, SysScalars* sc, in
     const double inv = 1.0 / sc[s].scale;
     x64[t].x += double(y32[t].x) * inv;
     x64[t].y += double(y32[t].y) * inv;
+  }
+}
+
+// undo the last correction x64 += y32 / scale of the systems whose pass made things worse
+template <int SP>
+__global__ void k_rollback(double2* x64, const float2* y32, const SysScalars* sc, int nb) {
+  const long long N = (long long)nb * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = int(t % SP);
+    if (!sc[s].rollback) continue;
+    const double inv = 1.0 / sc[s].scale;
+    x64[t].x -= double(y32[t].x) * inv;
+    x64[t].y -= double(y32[t].y) * inv;
   }
 }
 
@@ -1009,7 +1036,7 @@ struct mtcocr_handle {
   DevBuf omega, scal, nrun, nrun2, part;
   DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
   // COCR work vectors (type depends on the precision of the last solve)
-  DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vda, vdn, vx, vb;
+  DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vb;
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
   int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
@@ -1043,7 +1070,7 @@ struct mtcocr_handle {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
                            &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &b64, &x64,
-                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vda, &vdn,
+                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn,
                            &vx, &vb};
     int64_t t = 0;
     for (auto* b : all) t += (int64_t)b->bytes;
@@ -1168,7 +1195,8 @@ struct Engine {
   mtcocr_handle* h;
   EllView<double> A;
   GView<T> G;
-  T2 *x, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *da, *dn;
+  T2 *x, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *dn;
+  DiagA da;
   SysScalars* sc;
   int* nrun;
   double* part;
@@ -1191,7 +1219,9 @@ struct Engine {
     ym = h->vym.as<T2>();
     sm = h->vsm.as<T2>();
     help = h->vhelp.as<T2>();
-    da = h->vda.as<T2>();
+    da.a = h->da64.as<double>();
+    da.b = h->db64.as<double>();
+    da.omega = h->omega.as<double>();
     dn = h->vdn.as<T2>();
     sc = h->scal.as<SysScalars>();
     nrun = h->nrun.as<int>();
@@ -1211,9 +1241,8 @@ struct Engine {
                              part);
   }
 
+  // D_N^-1 per system (D_A^-1 is evaluated inside the edge kernels)
   void setup_diagonals(hipStream_t st) {
-    const int g1 = std::min(4096, (int)(((long long)nb * SP + 255) / 256));
-    k_diag_a<T, SP><<<g1, 256, 0, st>>>(h->da64.as<double>(), h->db64.as<double>(), omega, da, nb);
     const int g2 = std::min(4096, (int)(((long long)m * SP + 255) / 256) + 1);
     k_diag_n<T, SP><<<g2, 256, 0, st>>>(ell_view<double>(h), g_view<double>(h), omega, dn);
   }
@@ -1259,8 +1288,8 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
   const size_t nv = (size_t)h->nb * sp * es;
   const size_t nm = (size_t)std::max(h->m, 1) * sp * es;
   int rc = 0;
-  DevBuf* vecs[] = {&h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw, &h->vym, &h->vsm, &h->vda,
-                    &h->vx, &h->vb};
+  DevBuf* vecs[] = {&h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw, &h->vym, &h->vsm, &h->vx,
+                    &h->vb};
   for (DevBuf* b : vecs)
     if ((rc = ensure(h, *b, nv))) return rc;
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
@@ -1439,6 +1468,7 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
                                                 inner_floor, pass == 0 ? 1 : 0, max_outer);
       HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
+      k_rollback<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), E.ym, E.sc, h->nb);
       if (h->h_flags[0] == 0) break;
       k_to_inner<SP><<<gv, 256, 0, st>>>(h->r64.as<double2>(), E.b, E.x, E.sc, h->nb);
       E.prologue(st, maxiter);
@@ -1947,17 +1977,24 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
             const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : E.grid_e;
             if (grid == last_grid) continue;  // the cap no longer limits this grid
             last_grid = grid;
+            // each timed unit is the kernel plus the one-block reduction of its partials
+            double* probe_out = h->r64.as<double>();  // scratch: the outer residual buffer
             auto launch = [&]() {
-              if (fam == 0)
+              if (fam == 0) {
                 E.template spmv<0>(st, E.s, E.z, (const T2*)nullptr, (T2*)nullptr,
                                    (T2*)nullptr);
-              else if (fam == 1)
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_s,
+                                                                    probe_out);
+              } else if (fam == 1) {
                 k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
                                                                     E.sc, E.nrun);
-              else
+              } else {
                 k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
                                                                    E.da, E.help, E.sc, E.nrun,
                                                                    E.part);
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e,
+                                                                    probe_out);
+              }
             };
             launch();
             HIPC(hipGetLastError());
