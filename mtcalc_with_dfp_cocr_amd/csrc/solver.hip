@@ -855,6 +855,34 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
   if (threadIdx.x == 0) count_running(sc, SP, nrun);
 }
 
+// First stage of the partial-sum reduction for wide grids: workgroup g folds partial rows
+// [g*kFold, (g+1)*kFold) into out[g*P + q], in a fixed order, so that the one-workgroup scalar
+// kernels read nblk/kFold rows instead of nblk (one CU cannot stream megabytes of partials
+// quickly). Deterministic like the single-stage sum, with a different grouping.
+constexpr int kFold = 64;
+constexpr int kFoldAbove = 2048;  // grids up to this size are reduced in one stage
+
+template <int P>
+__global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, int nblk,
+                                              double* __restrict__ out) {
+  constexpr int R = 256 / P > 0 ? 256 / P : 1;  // partial rows handled side by side
+  __shared__ double sh[256];
+  const int g = blockIdx.x;
+  const int q = threadIdx.x % P, j = threadIdx.x / P;
+  const int b0 = g * kFold, b1 = min(b0 + kFold, nblk);
+  double acc = 0.0;
+  if (threadIdx.x < R * P)
+#pragma unroll 4
+    for (int b = b0 + j; b < b1; b += R) acc += part[(size_t)b * P + q];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < P) {
+    double t = 0.0;
+    for (int jj = 0; jj < R; ++jj) t += sh[jj * P + threadIdx.x];
+    out[(size_t)g * P + threadIdx.x] = t;
+  }
+}
+
 // the partial-sum reduction of a scalar kernel alone (the autotuner times kernel + reduction)
 template <int P>
 __global__ __launch_bounds__(kScalarThreads) void s_reduce_probe(const double* part, int nblk,
@@ -1033,7 +1061,7 @@ struct mtcocr_handle {
 
   // batch state
   int nsys = 0, sp = 0;
-  DevBuf omega, scal, nrun, nrun2, part;
+  DevBuf omega, scal, nrun, nrun2, part, part2;
   DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vb;
@@ -1069,7 +1097,7 @@ struct mtcocr_handle {
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
                            &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
-                           &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &b64, &x64,
+                           &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn,
                            &vx, &vb};
     int64_t t = 0;
@@ -1199,7 +1227,7 @@ struct Engine {
   DiagA da;
   SysScalars* sc;
   int* nrun;
-  double* part;
+  double *part, *part2;
   const double* omega;
   int nb, m, grid_s, grid_e, grid_n, layout;
 
@@ -1226,12 +1254,26 @@ struct Engine {
     sc = h->scal.as<SysScalars>();
     nrun = h->nrun.as<int>();
     part = h->part.as<double>();
+    part2 = h->part2.as<double>();
     omega = h->omega.as<double>();
     const int k = mtcocr_handle::sp_index(SP), q = std::is_same<T, float>::value ? 1 : 0;
     grid_s = grid_for((long long)nb * SP, h->cap_spmv[q][k]);
     grid_e = grid_for((long long)nb * SP, h->cap_vec[q][k]);
     grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
     layout = h->spmv_layout[q][k];
+  }
+
+  // partials of a grid of nblk workgroups, P values each, as the scalar kernels read them:
+  // folded by k_fold into part2 first when the grid is wide
+  template <int P>
+  const double* partials(hipStream_t st, int nblk, int& nread) {
+    if (nblk <= kFoldAbove) {
+      nread = nblk;
+      return part;
+    }
+    nread = (nblk + kFold - 1) / kFold;
+    k_fold<P><<<nread, 256, 0, st>>>(part, nblk, part2);
+    return part2;
   }
 
   template <int MODE, class TV>
@@ -1250,7 +1292,9 @@ struct Engine {
   // stages 1-4 of the request order: r = b - A x, X0 test, s = M^-1 r, p = a = w = 0, z = A s
   void prologue(hipStream_t st, int maxiter) {
     spmv<1>(st, x, r, b, ym, sm);
-    s_prologue<SP><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun);
+    int nr;
+    const double* pp = partials<SP * 2>(st, grid_s, nr);
+    s_prologue<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     k_pc_node<T, SP, false><<<grid_n, kBlock, 0, st>>>(G, r, nullptr, dn, help, sc, nrun);
     k_pc_edge_plain<T, SP><<<grid_e, kBlock, 0, st>>>(G, r, da, help, s, sc, nrun);
     const long long nv = (long long)nb * SP;
@@ -1259,20 +1303,26 @@ struct Engine {
     k_fill<T2><<<gf, 256, 0, st>>>(a, nv);
     k_fill<T2><<<gf, 256, 0, st>>>(w, nv);
     spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
-    s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
+    pp = partials<SP * 2>(st, grid_s, nr);
+    s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
 
   // one COCR iteration (stages 5-8)
   void iteration(hipStream_t st, int maxiter) {
     k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
     k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, s, p, w, da, help, sc, nrun, part);
-    s_alpha<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    int nr;
+    const double* pp = partials<SP * 2>(st, grid_e, nr);
+    s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     k_xr_mrs1<T, SP><<<grid_e, kBlock, 0, st>>>(x, p, r, a, sm, sc, nrun, part, nb);
-    s_mrs<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc);
+    pp = partials<SP * 2>(st, grid_e, nr);
+    s_mrs<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc);
     k_mrs2<T, SP><<<grid_e, kBlock, 0, st>>>(x, r, ym, sm, s, w, sc, nrun, part, nb);
-    s_stop<SP><<<1, kScalarThreads, 0, st>>>(part, grid_e, sc, nrun);
+    pp = partials<SP>(st, grid_e, nr);
+    s_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
-    s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(part, grid_s, sc, nrun, maxiter);
+    pp = partials<SP * 2>(st, grid_s, nr);
+    s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
 };
 
@@ -1464,7 +1514,9 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
       launch_spmv<double, SP, 2>(h, E.layout, E.grid_s, st, E.omega, h->x64.as<double2>(),
                                  h->r64.as<double2>(), h->b64.as<double2>(), nullptr, nullptr,
                                  E.sc, nrun_outer, E.part);
-      s_outer<SP><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_s, E.sc, nrun_outer, E.nrun, eps,
+      int nr;
+      const double* pp = E.template partials<SP * 2>(st, E.grid_s, nr);
+      s_outer<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, E.sc, nrun_outer, E.nrun, eps,
                                                 inner_floor, pass == 0 ? 1 : 0, max_outer);
       HIPC(hipMemcpyAsync(h->h_flags, nrun_outer, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
@@ -1549,6 +1601,8 @@ int set_batch(mtcocr_handle* h, int nsys, const double* omega) {
   if ((rc = ensure(h, h->nrun, 64))) return rc;
   if ((rc = ensure(h, h->nrun2, 64))) return rc;
   if ((rc = ensure(h, h->part, sizeof(double) * (size_t)kMaxBlocks * 64 * 2))) return rc;
+  if ((rc = ensure(h, h->part2, sizeof(double) * (size_t)(kMaxBlocks / kFold) * 64 * 2)))
+    return rc;
   const size_t v64 = (size_t)h->nb * sp * sizeof(double2);
   if ((rc = ensure(h, h->b64, v64))) return rc;
   if ((rc = ensure(h, h->x64, v64))) return rc;
@@ -1983,8 +2037,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
               if (fam == 0) {
                 E.template spmv<0>(st, E.s, E.z, (const T2*)nullptr, (T2*)nullptr,
                                    (T2*)nullptr);
-                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_s,
-                                                                    probe_out);
+                int nr;
+                const double* pp = E.template partials<SP * 2>(st, E.grid_s, nr);
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               } else if (fam == 1) {
                 k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
                                                                     E.sc, E.nrun);
@@ -1992,8 +2047,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
                                                                    E.da, E.help, E.sc, E.nrun,
                                                                    E.part);
-                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(E.part, E.grid_e,
-                                                                    probe_out);
+                int nr;
+                const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               }
             };
             launch();

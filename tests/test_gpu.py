@@ -279,3 +279,38 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
         xr = B.get_solution()
         for k in range(len(cases)):
             assert st[k]["req"] == 0 and rel(xr[k], x0[k]) <= 1e-5
+
+
+def test_wide_grid_two_stage_reduction(O, M, c1_model, c1_mesh):
+    """16 systems on 45000 edges: at caps above 2048 workgroups the dot-product partials are
+    folded in two stages. Same solution to rounding as the one-stage grid, deterministic, and
+    the FP64 oracle's iteration counts on two of the systems."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("two-stage partial reduction: 16 systems")
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    with Solver(0) as B:
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        B.set_launch(1024, 1024, 1024, "rowpack")
+        x1, s1 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        B.set_launch(16384, 16384, 16384, "rowpack")
+        x2, s2 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        x3, s3 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+    assert np.array_equal(x2, x3)
+    for k in range(len(cases)):
+        assert s1[k]["req"] == 0 and s2[k]["req"] == 0
+        assert abs(s1[k]["iter"] - s2[k]["iter"]) <= 2 + 0.02 * s1[k]["iter"]
+        assert rel(x2[k], x1[k]) <= 1e-5
+    for k in (0, 15):
+        f, a = cases[k]
+        xo, so, _ = O.pcocr(M.assemble(c1_model, f, a), c1_mesh, eps=1e-8)
+        say(f"case {f:.3g} Hz alpha={a}: wide {s2[k]['iter']} it, oracle {so['iter']} it")
+        assert abs(s2[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(x2[k], xo) <= 1e-5
