@@ -82,6 +82,61 @@ def cpu_baseline(M, m, ma, case, iters_needed, budget_iters, threads):
     return est, per_iter, setup, stn["seconds"] + st1["seconds"]
 
 
+def measure_traffic(args, sp):
+    """HBM bytes per launch of the batched FP64 SpMV: two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE) over this script's --kernels-only mode, each scaled by what the counter
+    reads for the 1 GiB copy kernel k_copy (known read and write bytes). None when rocprofv3
+    is unavailable or a pass fails."""
+    import csv
+    import re
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    pat = re.compile(r"k_spmv(_rg)?<double, %d, 0>" % sp)
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--kernels-only", "--no-mixed",
+               "--spmv-reps", "5", "--tune-reps", "2", "--nx", str(args.nx), "--ny",
+               str(args.ny), "--nz", str(args.nz), "--nfreq", str(args.nfreq), "--seed",
+               str(args.seed)]
+        log(f"traffic pass {ctr}")
+        env = dict(os.environ, TMPDIR="/tmp")
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                             stderr=subprocess.DEVNULL, start_new_session=True)
+        try:
+            rc = p.wait(timeout=240)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            log(f"traffic pass {ctr} timed out")
+            return None
+        files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs
+                 if f.endswith("counter_collection.csv")]
+        if rc != 0 or not files:
+            log(f"traffic pass {ctr} failed (rc={rc})")
+            return None
+        rows = sorted(csv.DictReader(open(files[0])), key=lambda r: int(r["Dispatch_Id"]))
+        spmv = [float(r["Counter_Value"]) for r in rows if pat.search(r["Kernel_Name"])
+                and r["Counter_Name"] == ctr]
+        copy = [float(r["Counter_Value"]) for r in rows if "k_copy" in r["Kernel_Name"]
+                and r["Counter_Name"] == ctr]
+        shutil.rmtree(d, ignore_errors=True)
+        if len(spmv) < 5 or not copy:
+            return None
+        kib = float(np.mean(spmv[-5:]))           # the timed launches at the tuned shape
+        factor = (1 << 30) / (float(np.mean(copy)) * 1024.0)
+        res[ctr] = {"kib_per_dispatch": kib, "calibration": factor,
+                    "bytes": kib * 1024.0 * factor}
+    return res
+
+
 def timed(S, prec, args):
     sync()
     t0 = time.perf_counter()
@@ -111,6 +166,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="launches per candidate shape in the autotuner")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure SpMV HBM traffic")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling mode: autotune, then only the batched SpMV (--spmv-reps "
                          "launches) and the 1 GiB copy calibration kernel; prints one line")
@@ -228,6 +285,10 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     spb = 1 << (nsys - 1).bit_length()
+    traffic = None if args.no_traffic else measure_traffic(args, spb)
+    log(f"spmv traffic: {traffic}")
+    traffic_bytes = (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]
+                     if traffic else None)
     out = {
         "metric": METRIC,
         "value": value,
@@ -247,7 +308,8 @@ def main():
                    "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU",
                    "launch": tune},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic_bytes,
+                     "traffic_counters": traffic,
                      "kernel": f"k_spmv<double,{spb},0> (batched FP64 SpMV)",
                      "algorithmic_bytes": spmv_bytes, "avg_ms": spmv_ms},
         "cpu_baseline": cpu,

@@ -1560,7 +1560,7 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
     for (int s = 0; s < h->nsys; ++s) {
       if (!fall[s]) continue;
       mtcocr_stats& o = stats[s];
-      o.req = fs[s].req == MTCOCR_OK && fs[s].true_rel_res >= eps ? MTCOCR_ERR_NOTCONV : fs[s].req;
+      o.req = fs[s].req;  // the FP64 solver's verdict, as for a plain FP64 solve
       o.iter += fs[s].iter;
       o.flags |= MTCOCR_FLAG_FP64_FINISH;
       o.residual_rel = fs[s].residual_rel;
@@ -1918,15 +1918,16 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
   });
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
-  // algorithmic bytes: FP64 matrix values + int32 column indices once per launch, then x read
-  // once and y written once per system (FP64 or FP32 vectors), plus the slice descriptors
+  // algorithmic bytes (independent of the layout): FP64 matrix values (16 B per complex,
+  // 8 B per real non-zero) and int32 column indices once per launch, int32 row pointers, then
+  // x read once and y written once per system (FP64 or FP32 vectors)
   const double ev = (precision == MTCOCR_PREC_FP64) ? 8.0 : 4.0;
   const double nnzc = double(h->nnz_c), nnzr = double(h->nnz_r);
   const double val = 2.0 * 8.0 * nnzc + 8.0 * nnzr;
   const double idx = 4.0 * (nnzc + nnzr);
   const double vec = double(nsys) * 2.0 * (2.0 * ev * h->nb);
-  const double desc = 16.0 * h->nslices;
-  *bytes_per_launch = val + idx + vec + desc;
+  const double ptrs = 4.0 * (h->nb + 1.0);
+  *bytes_per_launch = val + idx + vec + ptrs;
   return 0;
 }
 
