@@ -99,6 +99,25 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
                              const int* edges, const int* ig_t, const int* jg_t,
                              const double* gg_t, const int* is_node_bound);
 
+/* Locality order (no counterpart in the reference, which keeps RegularMeshBuilder's
+ * lexicographic edge numbering, Code/RegularMeshBuilder/main.cpp:923-1031): derives from a
+ * structured hexahedral mesh (cells nver[kpar*14] in the reference's vertex order, edges[2*n]
+ * 0-based node pairs, nodes < n_nodes) a device order of the edges in pencils of block x block
+ * nodes swept along z, and of the interior nodes after them. The matrix, the gradient and every
+ * vector are held in that order on the device; inputs and outputs keep the caller's order.
+ * Must precede set_matrix_* / set_gradient_mesh (MTCOCR_ERR_STATE otherwise); block = 0 restores
+ * the caller's order. Per-row arithmetic is unchanged (each row keeps its entry order), dot
+ * products regroup. Returns MTCOCR_ERR_ARG, keeping the caller's order, for a mesh that is not
+ * structured hexahedral. mtcocr_pcocr_2x2_folded and the COCR_FP executable apply it with
+ * block 16. */
+int mtcocr_set_mesh_order(mtcocr_handle* h, int kpar, int n_nodes, const int* nver, int n_edges,
+                          const int* edges, int block);
+
+/* The same order on the host, without a device: order[i] = caller's edge index of device row i
+ * (n_edges entries). */
+int mtcocr_mesh_order(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                      int block, int* order);
+
 /* Solve one system set by mtcocr_set_matrix_block2x2. pr = b (2*nb), x in/out. */
 int mtcocr_solve(mtcocr_handle* h, const double* pr, double* x, double eps, int maxiter,
                  int precision, mtcocr_stats* stats);

@@ -59,6 +59,13 @@ class Solver:
             pass
 
     # ---- set-up -------------------------------------------------------------------------
+    def set_mesh_order(self, ma: MeshArrays, block: int = 16) -> None:
+        """Hold the matrix, gradient and vectors in the mesh's pencil order on the device
+        (inputs and outputs keep the caller's order). Call before set_matrix / set_gradient."""
+        check(self._L.mtcocr_set_mesh_order(self._h, int(ma.kpar), int(ma.kuzlov),
+                                            _i32(ma.nver), int(len(ma.edges) // 2),
+                                            _i32(ma.edges), int(block)), "mtcocr_set_mesh_order")
+
     def set_matrix(self, s: BlockSystem):
         self._keep = (_i32(s.ig), _i32(s.jg), _i32(s.idi), _i32(s.ijg), _f64(s.di), _f64(s.gg))
         check(self._L.mtcocr_set_matrix_block2x2(self._h, int(s.nb), *self._keep),
@@ -174,6 +181,16 @@ class Solver:
                                          C.byref(d)), "mtcocr_matrix_info")
         return {"nnz_complex": a.value, "nnz_real": b.value, "m": c.value,
                 "device_bytes": d.value}
+
+
+def mesh_order(ma: MeshArrays, block: int = 16) -> np.ndarray:
+    """The device's locality order of a structured hexahedral mesh: order[i] = edge index of
+    device row i (pencils of block x block nodes swept along z)."""
+    n = len(ma.edges) // 2
+    order = np.empty(n, np.int32)
+    check(lib().mtcocr_mesh_order(int(ma.kpar), int(ma.kuzlov), _i32(ma.nver), n,
+                                  _i32(ma.edges), int(block), order), "mtcocr_mesh_order")
+    return order
 
 
 def pcocr_2x2_folded(n, ig, jg, idi, ijg, di, gg, pr, x, eps, maxiter, kpar, n_edges_c,

@@ -240,6 +240,13 @@ int set_gradient(mtcocr_handle* h, Inputs& in) {
                                   in.tsize3d ? in.gg_t.data() : nullptr, in.is_node_bound.data());
 }
 
+// the device's locality order of the mesh (kept as given when the mesh is not a structured
+// hexahedral one or has hanging nodes)
+void set_order(mtcocr_handle* h, Inputs& in) {
+  if (in.tsize3d) return;
+  (void)mtcocr_set_mesh_order(h, in.kpar, in.kuzlov, in.nver.data(), in.n_c, in.edges.data(), 16);
+}
+
 int run_single(const std::string& dir, bool mixed, bool strict) {
   auto t0 = std::chrono::steady_clock::now();
   std::ofstream log(join(dir, "logharm3dCalc"));
@@ -259,6 +266,7 @@ int run_single(const std::string& dir, bool mixed, bool strict) {
     std::fprintf(stderr, "COCR_FP: %s\n", mtcocr_error_string(rc));
     return 1;
   }
+  set_order(h, in);
   rc = mtcocr_set_matrix_block2x2(h, in.n_c, in.ig.data(), in.jg.data(), in.idi.data(),
                                   in.ijg.data(), in.di.data(), in.gg.data());
   if (!rc) rc = set_gradient(h, in);
@@ -342,6 +350,7 @@ int run_batch(const std::vector<std::string>& dirs, bool mixed, bool strict) {
   if (const char* e = std::getenv("MTCOCR_DEVICE")) dev = std::atoi(e);
   mtcocr_handle* h = nullptr;
   int rc = mtcocr_create(dev, &h);
+  if (!rc) set_order(h, ins[0]);
   if (!rc) rc = mtcocr_set_matrix_ks(h, nb, a.ig.data(), a.jg.data(), kd.data(), kg.data(),
                                      sd.data(), sg.data());
   if (!rc) rc = set_gradient(h, ins[0]);

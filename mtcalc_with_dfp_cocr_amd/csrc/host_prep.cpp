@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 
 #include "mtcocr.h"
 
@@ -22,7 +23,7 @@ struct Entry {
 // Code/COCR_FP/FoldedPreconditioner.cpp:653-736).
 template <class GetLower, class GetDiag>
 int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag diag,
-               SlicedEll& out) {
+               const int* order, SlicedEll& out) {
   if (nb <= 0 || !ig || !jg) return MTCOCR_ERR_ARG;
   const int64_t nnzl = ig[nb];
   std::vector<int64_t> cnt(nb + 1, 0);
@@ -59,6 +60,29 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
       ent[fill[jg[k]]++] = e;
     }
   (void)nnzl;
+  // Device row i holds the reference's row order[i] (identity without an order); columns are
+  // renumbered through the inverse and every row keeps the reference's entry order, so each
+  // row's sum is formed in the same order whatever the permutation.
+  std::vector<int> inv;
+  if (order) {
+    inv.assign(nb, -1);
+    for (int i = 0; i < nb; ++i) {
+      const int o = order[i];
+      if (o < 0 || o >= nb || inv[o] != -1) return MTCOCR_ERR_ARG;
+      inv[o] = i;
+    }
+  }
+  auto old_of = [&](int i) { return order ? order[i] : i; };
+  auto new_col = [&](int c) { return order ? inv[c] : c; };
+  {
+    std::vector<double> da(nb), db(nb);
+    for (int i = 0; i < nb; ++i) {
+      da[i] = out.diag_a[old_of(i)];
+      db[i] = out.diag_b[old_of(i)];
+    }
+    out.diag_a.swap(da);
+    out.diag_b.swap(db);
+  }
   // slices of 64 rows: per slice the maximum complex / real counts (explicit zeros dropped,
   // the diagonal always kept)
   const int ns = (nb + 63) / 64;
@@ -69,11 +93,14 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
   out.nc.assign(ns, 0);
   out.nr.assign(ns, 0);
   std::vector<int> rc(nb, 0), rr(nb, 0);
+  std::vector<char> diag_cplx(nb, 0);
   out.nnz_complex = out.nnz_real = 0;
   for (int i = 0; i < nb; ++i) {
-    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int oi = old_of(i);
+    for (int64_t k = ptr[oi]; k < ptr[oi + 1]; ++k) {
       const Entry& e = ent[k];
-      const bool keep = (e.col == i) || e.a != 0.0 || e.b != 0.0;
+      if (e.col == oi) diag_cplx[i] = e.cplx;
+      const bool keep = (e.col == oi) || e.a != 0.0 || e.b != 0.0;
       if (!keep) continue;
       if (e.cplx) ++rc[i]; else ++rr[i];
     }
@@ -94,10 +121,6 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
   // row pack: the diagonal (always stored, complex or real) is left out
   out.rp.assign(nb + 1, 0);
   out.rb.assign(nb + 1, 0);
-  std::vector<char> diag_cplx(nb, 0);
-  for (int i = 0; i < nb; ++i)
-    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
-      if (ent[k].col == i) diag_cplx[i] = ent[k].cplx;
   for (int i = 0; i < nb; ++i) {
     const int offc = rc[i] - (diag_cplx[i] ? 1 : 0), offr = rr[i] - (diag_cplx[i] ? 0 : 1);
     out.rp[i + 1] = out.rp[i] + offc + offr;
@@ -113,32 +136,33 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
   out.colr.assign(totr, 0);
   out.ar.assign(totr, 0.0);
   for (int i = 0; i < nb; ++i) {
+    const int oi = old_of(i);
     const int s = i >> 6, lane = i & 63;
     int kc = 0, kr = 0;
-    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+    for (int64_t k = ptr[oi]; k < ptr[oi + 1]; ++k) {
       const Entry& e = ent[k];
-      const bool keep = (e.col == i) || e.a != 0.0 || e.b != 0.0;
+      const bool keep = (e.col == oi) || e.a != 0.0 || e.b != 0.0;
       if (!keep) continue;
       if (e.cplx) {
         const int64_t p = out.offc[s] + 64LL * kc + lane;
-        out.colc[p] = e.col;
+        out.colc[p] = new_col(e.col);
         out.ac[p] = e.a;
         out.bc[p] = e.b;
         ++kc;
       } else {
         const int64_t p = out.offr[s] + 64LL * kr + lane;
-        out.colr[p] = e.col;
+        out.colr[p] = new_col(e.col);
         out.ar[p] = e.a;
         ++kr;
       }
     }
-    // row pack: complex off-diagonals, then real off-diagonals (each ascending)
+    // row pack: complex off-diagonals, then real off-diagonals (each in the row's order)
     int64_t q = out.rp[i], qb = out.rb[i];
     for (int pass = 0; pass < 2; ++pass)
-      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      for (int64_t k = ptr[oi]; k < ptr[oi + 1]; ++k) {
         const Entry& e = ent[k];
-        if (e.col == i || (e.a == 0.0 && e.b == 0.0) || e.cplx != (pass == 0)) continue;
-        out.pcol[q] = e.col;
+        if (e.col == oi || (e.a == 0.0 && e.b == 0.0) || e.cplx != (pass == 0)) continue;
+        out.pcol[q] = new_col(e.col);
         out.pa[q++] = e.a;
         if (pass == 0) out.pb[qb++] = e.b;
       }
@@ -153,7 +177,7 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
 }  // namespace
 
 int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, const int* ijg,
-                      const double* di, const double* gg, SlicedEll& out) {
+                      const double* di, const double* gg, const int* order, SlicedEll& out) {
   if (!idi || !ijg || !di || !gg) return MTCOCR_ERR_ARG;
   auto lower = [&](int64_t k) {
     Entry e;
@@ -173,11 +197,11 @@ int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, cons
     e.col = i;
     return e;
   };
-  return build_rows(nb, ig, jg, lower, diag, out);
+  return build_rows(nb, ig, jg, lower, diag, order, out);
 }
 
 int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const double* k_gg,
-                const double* s_di, const double* s_gg, SlicedEll& out) {
+                const double* s_di, const double* s_gg, const int* order, SlicedEll& out) {
   if (!k_di || !k_gg || !s_di || !s_gg) return MTCOCR_ERR_ARG;
   auto lower = [&](int64_t k) {
     Entry e;
@@ -195,12 +219,13 @@ int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const 
     e.col = i;
     return e;
   };
-  return build_rows(nb, ig, jg, lower, diag, out);
+  return build_rows(nb, ig, jg, lower, diag, order, out);
 }
 
 int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
                    const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
-                   const double* gg_t, const int* is_node_bound, GradientEll& out) {
+                   const double* gg_t, const int* is_node_bound, const int* edge_order,
+                   GradientEll& out) {
   if (kpar <= 0 || n_edges <= 0 || n_nodes_c <= 0 || !nvkat || !nver || !sigma3d || !edges ||
       !is_node_bound)
     return MTCOCR_ERR_ARG;
@@ -265,6 +290,27 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
       gv[fill[node]] = tval[k];
       fill[node]++;
     }
+  // Device order: edges as the matrix rows (edge_order, new -> old); interior nodes ranked by
+  // their first incident edge in that order, so that a node's edges and the node sit close.
+  // Entry order inside every row of G and G^T stays the reference's.
+  std::vector<int> einv, norder(m), ninv(m);
+  for (int i = 0; i < m; ++i) norder[i] = ninv[i] = i;
+  if (edge_order) {
+    einv.assign(n_edges, -1);
+    for (int e = 0; e < n_edges; ++e) {
+      const int o = edge_order[e];
+      if (o < 0 || o >= n_edges || einv[o] != -1) return MTCOCR_ERR_ARG;
+      einv[o] = e;
+    }
+    std::vector<int> first(m, n_edges);
+    for (int i = 0; i < m; ++i)
+      for (int64_t q = gptr[i]; q < gptr[i + 1]; ++q) first[i] = std::min(first[i], einv[gc[q]]);
+    std::stable_sort(norder.begin(), norder.end(),
+                     [&](int a, int b) { return first[a] < first[b]; });
+    for (int i = 0; i < m; ++i) ninv[norder[i]] = i;
+  }
+  auto new_edge = [&](int e) { return edge_order ? einv[e] : e; };
+  auto old_edge = [&](int e) { return edge_order ? edge_order[e] : e; };
   out.n = n_edges;
   out.m = m;
   out.wg = 0;
@@ -274,22 +320,121 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
   out.gcol.assign((size_t)out.wg * m, 0);
   out.gval.assign((size_t)out.wg * m, 0.0);
   for (int i = 0; i < m; ++i) {
+    const int oi = norder[i];
     int k = 0;
-    for (int64_t q = gptr[i]; q < gptr[i + 1]; ++q, ++k) {
-      out.gcol[(size_t)k * m + i] = gc[q];
+    for (int64_t q = gptr[oi]; q < gptr[oi + 1]; ++q, ++k) {
+      out.gcol[(size_t)k * m + i] = new_edge(gc[q]);
       out.gval[(size_t)k * m + i] = gv[q];
     }
   }
   out.gtcol.assign((size_t)out.wgt * n_edges, 0);
   out.gtval.assign((size_t)out.wgt * n_edges, 0.0);
   for (int e = 0; e < n_edges; ++e) {
+    const int oe = old_edge(e);
     int k = 0;
-    for (int64_t q = tptr[e]; q < tptr[e + 1]; ++q, ++k) {
-      out.gtcol[(size_t)k * n_edges + e] = tcol[q];
+    for (int64_t q = tptr[oe]; q < tptr[oe + 1]; ++q, ++k) {
+      out.gtcol[(size_t)k * n_edges + e] = ninv[tcol[q]];
       out.gtval[(size_t)k * n_edges + e] = tval[q];
     }
   }
   return 0;
 }
 
+int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                    int block, std::vector<int>& order) {
+  if (kpar <= 0 || n_nodes <= 0 || n_edges <= 0 || !nver || !edges || block < 1)
+    return MTCOCR_ERR_ARG;
+  // integer node coordinates from the cells' local vertex order (vertex l of a cell sits at
+  // offset (l&1, (l>>1)&1, (l>>2)&1) from vertex 0), propagated cell to cell
+  const int kUnset = std::numeric_limits<int>::min();
+  std::vector<int> cx(n_nodes, kUnset), cy(n_nodes, kUnset), cz(n_nodes, kUnset);
+  std::vector<int64_t> nptr(n_nodes + 1, 0);
+  for (int64_t c = 0; c < kpar; ++c)
+    for (int l = 0; l < 8; ++l) {
+      const int v = nver[c * 14 + l];
+      if (v < 0 || v >= n_nodes) return MTCOCR_ERR_ARG;
+      nptr[v + 1]++;
+    }
+  for (int v = 0; v < n_nodes; ++v) nptr[v + 1] += nptr[v];
+  std::vector<int> ncell(nptr[n_nodes]);
+  {
+    std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int c = 0; c < kpar; ++c)
+      for (int l = 0; l < 8; ++l) ncell[fill[nver[(int64_t)c * 14 + l]]++] = c;
+  }
+  std::vector<char> done(kpar, 0);
+  std::vector<int> queue;
+  queue.reserve(kpar);
+  for (int seed = 0; seed < kpar; ++seed) {
+    if (done[seed]) continue;
+    const int v0 = nver[(int64_t)seed * 14];
+    if (cx[v0] == kUnset) cx[v0] = cy[v0] = cz[v0] = 0;
+    done[seed] = 1;
+    queue.push_back(seed);
+    for (size_t head = queue.size() - 1; head < queue.size(); ++head) {
+      const int c = queue[head];
+      const int* v = nver + (int64_t)c * 14;
+      int bx = kUnset, by = 0, bz = 0;
+      for (int l = 0; l < 8 && bx == kUnset; ++l)
+        if (cx[v[l]] != kUnset) {
+          bx = cx[v[l]] - (l & 1);
+          by = cy[v[l]] - ((l >> 1) & 1);
+          bz = cz[v[l]] - ((l >> 2) & 1);
+        }
+      if (bx == kUnset) return MTCOCR_ERR_ARG;
+      for (int l = 0; l < 8; ++l) {
+        const int x = bx + (l & 1), y = by + ((l >> 1) & 1), z = bz + ((l >> 2) & 1);
+        if (cx[v[l]] == kUnset) {
+          cx[v[l]] = x;
+          cy[v[l]] = y;
+          cz[v[l]] = z;
+        } else if (cx[v[l]] != x || cy[v[l]] != y || cz[v[l]] != z) {
+          return MTCOCR_ERR_ARG;  // not a structured hexahedral mesh in this vertex order
+        }
+        for (int64_t q = nptr[v[l]]; q < nptr[v[l] + 1]; ++q)
+          if (!done[ncell[q]]) {
+            done[ncell[q]] = 1;
+            queue.push_back(ncell[q]);
+          }
+      }
+    }
+  }
+  // edge key: (pencil x, pencil y, z, y, x, direction) of the edge's lower end
+  struct Key {
+    int px, py, z, y, x, d;
+  };
+  std::vector<Key> key(n_edges);
+  for (int e = 0; e < n_edges; ++e) {
+    const int a = edges[(int64_t)e * 2], b = edges[(int64_t)e * 2 + 1];
+    if (a < 0 || b < 0 || a >= n_nodes || b >= n_nodes || cx[a] == kUnset || cx[b] == kUnset)
+      return MTCOCR_ERR_ARG;
+    const int x = std::min(cx[a], cx[b]), y = std::min(cy[a], cy[b]), z = std::min(cz[a], cz[b]);
+    const int d = cx[a] != cx[b] ? 0 : (cy[a] != cy[b] ? 1 : 2);
+    key[e] = Key{x >= 0 ? x / block : -1 - (-x - 1) / block, y >= 0 ? y / block : -1 - (-y - 1) / block,
+                 z, y, x, d};
+  }
+  order.resize(n_edges);
+  for (int e = 0; e < n_edges; ++e) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const Key &p = key[a], &q = key[b];
+    if (p.px != q.px) return p.px < q.px;
+    if (p.py != q.py) return p.py < q.py;
+    if (p.z != q.z) return p.z < q.z;
+    if (p.y != q.y) return p.y < q.y;
+    if (p.x != q.x) return p.x < q.x;
+    return p.d < q.d;
+  });
+  return 0;
+}
+
 }  // namespace mtcocr
+
+extern "C" int mtcocr_mesh_order(int kpar, int n_nodes, const int* nver, int n_edges,
+                                 const int* edges, int block, int* order) {
+  if (!order) return MTCOCR_ERR_ARG;
+  std::vector<int> ord;
+  const int rc = mtcocr::order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
+  if (rc) return rc;
+  std::memcpy(order, ord.data(), sizeof(int) * (size_t)n_edges);
+  return 0;
+}

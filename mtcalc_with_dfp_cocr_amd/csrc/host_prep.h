@@ -43,17 +43,29 @@ struct GradientEll {
   std::vector<double> gtval;
 };
 
-// From the reference's lower-triangle 2x2-block arrays (0-based). Returns 0 or a negative code.
+// From the reference's lower-triangle 2x2-block arrays (0-based). `order` (new -> old row,
+// may be null) permutes rows and columns symmetrically; every row keeps its entry order.
+// Returns 0 or a negative code.
 int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, const int* ijg,
-                      const double* di, const double* gg, SlicedEll& out);
+                      const double* di, const double* gg, const int* order, SlicedEll& out);
 
 // From a K/S pair, one real per block on the same portrait.
 int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const double* k_gg,
-                const double* s_di, const double* s_gg, SlicedEll& out);
+                const double* s_di, const double* s_gg, const int* order, SlicedEll& out);
 
-// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259).
+// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259), with the
+// edges in `edge_order` (new -> old, may be null) and the interior nodes ranked by their first
+// incident edge in that order.
 int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
                    const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
-                   const double* gg_t, const int* is_node_bound, GradientEll& out);
+                   const double* gg_t, const int* is_node_bound, const int* edge_order,
+                   GradientEll& out);
+
+// Locality order of the edges of a structured hexahedral mesh (cells as nver[kpar][14] with the
+// reference's vertex order, edges as node pairs): pencils of block x block nodes in x-y, each
+// swept along z, so that the rows a GPU works on together gather from a small, cache-resident
+// set of columns. Fails (negative code) when the cells are not a structured hexahedral mesh.
+int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                    int block, std::vector<int>& order);
 
 }  // namespace mtcocr

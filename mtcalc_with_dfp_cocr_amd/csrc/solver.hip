@@ -1046,6 +1046,9 @@ struct mtcocr_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* h_flags = nullptr;  // pinned: [0] running count
 
+  // device row order (new -> reference edge index); empty = the reference's order
+  std::vector<int> order;
+
   // matrix (shared index arrays, FP64 and FP32 values)
   bool have_matrix = false;
   bool ks_form = false;
@@ -1614,15 +1617,18 @@ int set_batch(mtcocr_handle* h, int nsys, const double* omega) {
 }
 
 // interleave nsys host vectors (each 2*nb doubles) into the [row][system] device layout
+// (device row r holds the caller's row order[r])
 int put_batch(mtcocr_handle* h, DevBuf& dst, const double* src) {
   const int sp = h->sp, nb = h->nb;
+  const int* ord = h->order.empty() ? nullptr : h->order.data();
   std::vector<double2> tmp((size_t)nb * sp);
   for (int r = 0; r < nb; ++r)
     for (int s = 0; s < sp; ++s) {
       double2 v;
+      const size_t o = ord ? (size_t)ord[r] : (size_t)r;
       if (s < h->nsys && src) {
-        v.x = src[(size_t)s * 2 * nb + 2 * r];
-        v.y = src[(size_t)s * 2 * nb + 2 * r + 1];
+        v.x = src[(size_t)s * 2 * nb + 2 * o];
+        v.y = src[(size_t)s * 2 * nb + 2 * o + 1];
       } else {
         v.x = 0.0;
         v.y = 0.0;
@@ -1639,11 +1645,14 @@ int get_batch(mtcocr_handle* h, const DevBuf& srcb, double* dst) {
   std::vector<double2> tmp((size_t)nb * sp);
   HIPC(hipMemcpyAsync(tmp.data(), srcb.p, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
-  for (int r = 0; r < nb; ++r)
+  const int* ord = h->order.empty() ? nullptr : h->order.data();
+  for (int r = 0; r < nb; ++r) {
+    const size_t o = ord ? (size_t)ord[r] : (size_t)r;
     for (int s = 0; s < h->nsys; ++s) {
-      dst[(size_t)s * 2 * nb + 2 * r] = tmp[(size_t)r * sp + s].x;
-      dst[(size_t)s * 2 * nb + 2 * r + 1] = tmp[(size_t)r * sp + s].y;
+      dst[(size_t)s * 2 * nb + 2 * o] = tmp[(size_t)r * sp + s].x;
+      dst[(size_t)s * 2 * nb + 2 * o + 1] = tmp[(size_t)r * sp + s].y;
     }
+  }
   return 0;
 }
 
@@ -1733,9 +1742,11 @@ int mtcocr_set_matrix_block2x2(mtcocr_handle* h, int nb, const int* ig, const in
                                const int* idi, const int* ijg, const double* di,
                                const double* gg) {
   if (!h || nb <= 0) return MTCOCR_ERR_ARG;
+  if (!h->order.empty() && (int)h->order.size() != nb) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
   SlicedEll E;
-  int rc = ell_from_block2x2(nb, ig, jg, idi, ijg, di, gg, E);
+  int rc = ell_from_block2x2(nb, ig, jg, idi, ijg, di, gg,
+                             h->order.empty() ? nullptr : h->order.data(), E);
   if (rc) return rc;
   return upload_matrix(h, E, false);
 }
@@ -1744,9 +1755,11 @@ int mtcocr_set_matrix_ks(mtcocr_handle* h, int nb, const int* ig, const int* jg,
                          const double* k_di, const double* k_gg, const double* s_di,
                          const double* s_gg) {
   if (!h || nb <= 0) return MTCOCR_ERR_ARG;
+  if (!h->order.empty() && (int)h->order.size() != nb) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
   SlicedEll E;
-  int rc = ell_from_ks(nb, ig, jg, k_di, k_gg, s_di, s_gg, E);
+  int rc = ell_from_ks(nb, ig, jg, k_di, k_gg, s_di, s_gg,
+                       h->order.empty() ? nullptr : h->order.data(), E);
   if (rc) return rc;
   return upload_matrix(h, E, true);
 }
@@ -1757,10 +1770,11 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
                              const double* gg_t, const int* is_node_bound) {
   if (!h) return MTCOCR_ERR_ARG;
   if (h->have_matrix && n_edges != h->nb) return MTCOCR_ERR_ARG;
+  if (!h->order.empty() && (int)h->order.size() != n_edges) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
   GradientEll G;
   int rc = build_gradient(kpar, n_edges, n_nodes_c, nvkat, nver, sigma3d, edges, ig_t, jg_t, gg_t,
-                          is_node_bound, G);
+                          is_node_bound, h->order.empty() ? nullptr : h->order.data(), G);
   if (rc) return rc;
   if (G.wg > 8) return MTCOCR_ERR_ARG;  // more than 8 edges at a node: not a hexahedral mesh
   hipStream_t st = h->stream;
@@ -1777,6 +1791,24 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
   h->wgt = G.wgt;
   h->have_g = true;
   h->work_prec = -1;
+  return 0;
+}
+
+int mtcocr_set_mesh_order(mtcocr_handle* h, int kpar, int n_nodes, const int* nver,
+                          int n_edges, const int* edges, int block) {
+  if (!h || block < 0) return MTCOCR_ERR_ARG;
+  if (h->have_matrix || h->have_g) return MTCOCR_ERR_STATE;  // the layouts are built in order
+  if (block == 0) {
+    h->order.clear();
+    return 0;
+  }
+  std::vector<int> ord;
+  const int rc = order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
+  if (rc) {
+    h->order.clear();
+    return rc;
+  }
+  h->order.swap(ord);
   return 0;
 }
 
@@ -1862,6 +1894,8 @@ int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi,
   mtcocr_handle* h = nullptr;
   int rc = mtcocr_create(dev, &h);
   if (rc) return rc;
+  // locality order from the mesh when it is a structured hexahedral one (else the reference's)
+  (void)mtcocr_set_mesh_order(h, kpar, n_nodes_c, nver, n_edges_c, edges, 16);
   rc = mtcocr_set_matrix_block2x2(h, n / 2, ig, jg, idi, ijg, di, gg);
   if (!rc)
     rc = mtcocr_set_gradient_mesh(h, kpar, n_edges_c, n_nodes_c, nvkat, nver, sigma3d, edges,
@@ -2110,7 +2144,10 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
   const int nb = h->nb;
   std::vector<double2> in((size_t)nb * sp);
   for (int r = 0; r < nb; ++r)
-    for (int s = 0; s < sp; ++s) in[(size_t)r * sp + s] = s == 0 ? make_double2(x[2 * r], x[2 * r + 1]) : make_double2(0, 0);
+    for (int s = 0; s < sp; ++s) {
+      const size_t o = h->order.empty() ? (size_t)r : (size_t)h->order[r];
+      in[(size_t)r * sp + s] = s == 0 ? make_double2(x[2 * o], x[2 * o + 1]) : make_double2(0, 0);
+    }
   std::vector<double2> out((size_t)nb * sp);
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
@@ -2144,8 +2181,9 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
   });
   if (rc) return rc;
   for (int r = 0; r < nb; ++r) {
-    y[2 * r] = out[(size_t)r * sp].x;
-    y[2 * r + 1] = out[(size_t)r * sp].y;
+    const size_t o = h->order.empty() ? (size_t)r : (size_t)h->order[r];
+    y[2 * o] = out[(size_t)r * sp].x;
+    y[2 * o + 1] = out[(size_t)r * sp].y;
   }
   return 0;
 }

@@ -184,7 +184,13 @@ def test_pcocr_2x2_folded_mirror(S, M, c1_system, c1_mesh):
     rc = pcocr_2x2_folded(2 * s.nb, s.ig, s.jg, s.idi, s.ijg, s.di, s.gg, s.pr, x, 1e-8, 10000,
                           ma.kpar, s.nb, ma.kuzlov, ma.xyz, ma.nvkat, ma.nver, ma.sig_flags,
                           ma.edges, is_node_bound=ma.is_node_bound, stats=info)
-    xh, st = S.solve(s.pr, eps=1e-8, precision="fp64")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    with Solver(0) as H:  # the one-shot call holds the mesh's pencil order (block 16)
+        H.set_mesh_order(ma, 16)
+        H.set_matrix(s)
+        H.set_gradient(ma)
+        xh, st = H.solve(s.pr, eps=1e-8, precision="fp64")
     assert rc == 0 and info["iter"] == st["iter"] and np.array_equal(x, xh)
 
 
@@ -314,3 +320,33 @@ def test_wide_grid_two_stage_reduction(O, M, c1_model, c1_mesh):
         say(f"case {f:.3g} Hz alpha={a}: wide {s2[k]['iter']} it, oracle {so['iter']} it")
         assert abs(s2[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
         assert rel(x2[k], xo) <= 1e-5
+
+
+@pytest.mark.parametrize("block", [4, 16])
+def test_mesh_order_parity(O, M, c1_system, c1_mesh, oracle_c1, block):
+    """Matrix, gradient and vectors held in the pencil order: SpMV and preconditioner at the
+    same tolerances, and the FP64 solve against the oracle, inputs and outputs in the caller's
+    order."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say(f"mesh order block {block}")
+    xo, so = oracle_c1
+    x = np.random.default_rng(14).standard_normal(2 * c1_system.nb)
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, block)
+        B.set_matrix(c1_system)
+        B.set_gradient(c1_mesh)
+        ref = O.spmv(c1_system, x)
+        got = B.apply("spmv", x, "fp64")
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+        ref = O.Dfp(c1_mesh, c1_system).apply(x)
+        got = B.apply("precond", x, "fp64")
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+        xs, st = B.solve(c1_system.pr, eps=1e-8, maxiter=10000, precision="fp64")
+        say(f"ordered solve {st['iter']} it, oracle {so['iter']} it")
+        assert st["req"] == 0 and abs(st["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(xs, xo) <= 1e-5
+        with pytest.raises(RuntimeError):
+            B.set_mesh_order(c1_mesh, block)  # after the matrix: refused
