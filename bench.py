@@ -103,7 +103,8 @@ def measure_traffic(args, sp):
         d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--output-format", "csv", "--",
                sys.executable, os.path.abspath(__file__), "--kernels-only", "--no-mixed",
-               "--spmv-reps", "5", "--tune-reps", "2", "--nx", str(args.nx), "--ny",
+               "--spmv-reps", "5", "--tune-reps", "2", "--order-block",
+               str(args.order_block), "--nx", str(args.nx), "--ny",
                str(args.ny), "--nz", str(args.nz), "--nfreq", str(args.nfreq), "--seed",
                str(args.seed)]
         log(f"traffic pass {ctr}")
@@ -164,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--order-block", type=int, default=16,
+                    help="pencil width of the device's locality order (0: the file order)")
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="launches per candidate shape in the autotuner")
     ap.add_argument("--no-traffic", action="store_true",
@@ -184,6 +187,8 @@ def main():
 
     nsys = len(cases)
     S = Solver(0)
+    if args.order_block:
+        S.set_mesh_order(ma, args.order_block)
     S.set_matrix_ks(p, *ks)
     S.set_gradient(ma)
     log(f"device layout: {S.info()}")
@@ -306,6 +311,7 @@ def main():
                                f"{args.nz} cells (nb={p.nb}), {args.nfreq} freqs 1e-3..1e2 Hz x 2 "
                                f"pol = {nsys} systems, MRS stop at {args.eps:g}",
                    "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU",
+                   "order_block": args.order_block,
                    "launch": tune},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_bytes,

@@ -1042,6 +1042,7 @@ using namespace mtcocr;
 
 struct mtcocr_handle {
   int device = 0;
+  int n_cu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* h_flags = nullptr;  // pinned: [0] running count
@@ -1716,6 +1717,7 @@ int mtcocr_create(int device, mtcocr_handle** out) {
   HIPC(hipSetDevice(device));
   std::unique_ptr<mtcocr_handle> h(new mtcocr_handle());
   h->device = device;
+  HIPC(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device));
   HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPC(hipEventCreate(&h->ev0));
   HIPC(hipEventCreate(&h->ev1));
@@ -2058,8 +2060,32 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
       for (int fam = 0; fam < 3; ++fam) {
         const int nlayouts = (fam == 0 && SP >= 4) ? 2 : 1;
         for (int layout = 0; layout < nlayouts; ++layout) {
+          // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
+          // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
+          // twice that
+          int per_cu = 0;
+          if (fam == 0 && layout == 1) {
+            if constexpr (SP >= 4)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_rg<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0) {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_spmv<T, SP, 0>), kBlock, 0));
+          } else if (fam == 1) {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_pc_node<T, SP, true>), kBlock, 0));
+          } else {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_pc_edge_iter<T, SP>), kBlock, 0));
+          }
+          std::vector<int> cands(std::begin(kCands), std::end(kCands));
+          const int resident = round_up(std::max(1, per_cu * h->n_cu), kXcd);
+          for (int c : {resident, 2 * resident})
+            if (c <= kMaxBlocks) cands.push_back(c);
+          std::sort(cands.begin(), cands.end());
+          cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
           int last_grid = -1;
-          for (int cap : kCands) {
+          for (int cap : cands) {
             *caps[fam] = cap;
             if (fam == 0) h->spmv_layout[q][ki] = layout;
             Engine<T, SP> E(h);
