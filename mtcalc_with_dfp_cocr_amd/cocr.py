@@ -141,12 +141,13 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    LAYOUTS = {"ell": 0, "rowpack": 1}
+    LAYOUTS = {"ell": 0, "rowpack": 1, "rowpad": 2, "rowpad_lds8": 3, "rowpad_lds16": 4}
+    LAYOUT_NAMES = ("ell", "rowpack", "rowpad", "rowpad_lds8", "rowpad_lds16")
 
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
                    spmv_layout: str | None = None) -> None:
         """Workgroup caps of the SpMV / edge-streaming / node kernels (0 keeps a value) and the
-        SpMV layout ("ell" or "rowpack"; None keeps it)."""
+        SpMV layout (a key of LAYOUTS; None keeps it)."""
         lay = -1 if spmv_layout is None else self.LAYOUTS[spmv_layout]
         check(self._L.mtcocr_set_launch(self._h, int(cap_spmv), int(cap_vec), int(cap_node),
                                         lay), "mtcocr_set_launch")
@@ -156,7 +157,7 @@ class Solver:
         check(self._L.mtcocr_get_launch(self._h, PRECISION[precision], C.byref(a), C.byref(b),
                                         C.byref(c), C.byref(d)), "mtcocr_get_launch")
         return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
-                "spmv_layout": ("ell", "rowpack")[d.value]}
+                "spmv_layout": self.LAYOUT_NAMES[d.value]}
 
     def autotune(self, precision: str = "fp64", reps: int = 10) -> dict:
         """Time the tunable kernels of the current batch at each candidate shape and keep the
@@ -166,7 +167,7 @@ class Solver:
                                       C.byref(b), C.byref(c), C.byref(d), C.byref(ms)),
               "mtcocr_autotune")
         return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
-                "spmv_layout": ("ell", "rowpack")[d.value], "spmv_ms": ms.value}
+                "spmv_layout": self.LAYOUT_NAMES[d.value], "spmv_ms": ms.value}
 
     def apply(self, op: str, x, precision: str = "fp64"):
         """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""
