@@ -160,13 +160,11 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
 
 /* Launch shapes: workgroup caps of the grid-stride SpMV, edge-streaming and node kernels
  * (multiples of 8 in [8, 16384]; 0 keeps a value) and the SpMV layout (0 sliced ELL, 1 row
- * pack with one row per group of lanes, 2 the same padded to 32 entries per row (when no row
- * is longer; else 1), 3 / 4 the padded pack with the gathers landing in LDS in batches of 8 /
- * 16 (FP64 vectors; FP32 uses 2); 1-4 apply to batches of 4..64 systems; -1 keeps it).
+ * pack with one row per group of lanes, used for batches of 4..64 systems; -1 keeps it).
  * set_launch applies to both precisions and every batch width; get_launch reports those of
  * `precision` at the current batch width. Caps change only the reduction grouping of the dot
  * products and the layout only the summation order inside a row (results are deterministic for
- * a fixed shape). Defaults: caps 8192 / 2048 / 8192, padded row pack from 16 systems up. */
+ * a fixed shape). Defaults: caps 8192 / 2048 / 8192, row pack from 16 systems up. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,

@@ -141,8 +141,8 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    LAYOUTS = {"ell": 0, "rowpack": 1, "rowpad": 2, "rowpad_lds8": 3, "rowpad_lds16": 4}
-    LAYOUT_NAMES = ("ell", "rowpack", "rowpad", "rowpad_lds8", "rowpad_lds16")
+    LAYOUTS = {"ell": 0, "rowpack": 1}
+    LAYOUT_NAMES = ("ell", "rowpack")
 
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
                    spmv_layout: str | None = None) -> None:

@@ -171,32 +171,6 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
     for (; kr < out.nr[s]; ++kr) out.colr[out.offr[s] + 64LL * kr + lane] = i;
   }
   // lanes past nb in the last slice keep column 0 and zero values
-  // padded row pack
-  constexpr int W = SlicedEll::kRowWidth;
-  int wmax = 0, wbmax = 0;
-  for (int i = 0; i < nb; ++i) {
-    wmax = std::max(wmax, out.rp[i + 1] - out.rp[i]);
-    wbmax = std::max(wbmax, out.rb[i + 1] - out.rb[i]);
-  }
-  out.qcol.clear();
-  out.qa.clear();
-  out.qb.clear();
-  out.wb = 0;
-  if (wmax <= W && (int64_t)nb * W <= 2147483000LL) {
-    out.wb = std::max(wbmax, 1);
-    out.qcol.resize((size_t)nb * W);
-    out.qa.assign((size_t)nb * W, 0.0);
-    out.qb.assign((size_t)nb * out.wb, 0.0);
-    for (int i = 0; i < nb; ++i) {
-      const int n = out.rp[i + 1] - out.rp[i], nc = out.rb[i + 1] - out.rb[i];
-      for (int k = 0; k < W; ++k) {
-        const size_t q = (size_t)i * W + k;
-        out.qcol[q] = k < n ? out.pcol[out.rp[i] + k] : i;
-        out.qa[q] = k < n ? out.pa[out.rp[i] + k] : 0.0;
-      }
-      for (int k = 0; k < nc; ++k) out.qb[(size_t)i * out.wb + k] = out.pb[out.rb[i] + k];
-    }
-  }
   return 0;
 }
 

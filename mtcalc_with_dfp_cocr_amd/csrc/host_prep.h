@@ -30,15 +30,6 @@ struct SlicedEll {
   // columns), whose b values are [rb[i], rb[i+1]) of pb, then its real ones (ascending).
   std::vector<int> rp, rb, pcol;
   std::vector<double> pa, pb;
-  // Padded row pack: the same rows at a fixed width kRowWidth (entry k of row i at
-  // i * kRowWidth + k, complex entries first; padding = the row itself with value 0) and the b
-  // values at a fixed width wb (the largest complex count), so that every address follows from
-  // the row index. Built when no row has more than kRowWidth off-diagonals (hexahedral edge
-  // meshes have at most 32); empty otherwise.
-  static constexpr int kRowWidth = 32;
-  int wb = 0;
-  std::vector<int> qcol;
-  std::vector<double> qa, qb;
 };
 
 struct GradientEll {

@@ -79,14 +79,6 @@ struct RowView {
   int nb;
 };
 
-// padded row pack: entry k of row r at r * kRowWidth + k, b at r * wb + k (see host_prep.h)
-struct RowPadView {
-  const int* col;
-  const double *a, *b, *da, *db;
-  int nb, wb;
-};
-constexpr int kRowWidth = SlicedEll::kRowWidth;
-
 template <class T>
 struct GView {
   const int *gcol, *gtcol;
@@ -436,35 +428,6 @@ __device__ inline int grp_bcast(int v) {
   }
 }
 
-// grp_bcast with a value whose availability the broadcast must wait for: `after` becomes the
-// DPP "old" operand, which row broadcasts never select (every source lane is valid), so the
-// result is grp_bcast(v) while the compiler cannot schedule the broadcast (and hold its
-// result in registers) before `after` exists.
-template <int SP, int J>
-__device__ inline int grp_bcast_after(int v, int after) {
-  if constexpr (SP == 4) {
-    return __builtin_amdgcn_update_dpp(after, v, J | (J << 2) | (J << 4) | (J << 6), 0xF, 0xF,
-                                       false);
-  } else if constexpr (SP == 8) {
-    const int lo = __builtin_amdgcn_update_dpp(after, v, 0x150 + J, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(after, v, 0x150 + 8 + J, 0xF, 0xF, false);
-    return (threadIdx.x & 8) ? hi : lo;
-  } else if constexpr (SP == 16) {
-    return __builtin_amdgcn_update_dpp(after, v, 0x150 + J, 0xF, 0xF, false);
-  } else {
-    (void)after;
-    return grp_bcast<SP, J>(v);
-  }
-}
-
-template <int SP, int J>
-__device__ inline double grp_bcast_d_after(double v, int after) {
-  const long long u = __builtin_bit_cast(long long, v);
-  const int lo = grp_bcast_after<SP, J>(int(u & 0xffffffffLL), after);
-  const int hi = grp_bcast_after<SP, J>(int(u >> 32), after);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-
 template <int SP, int J>
 __device__ inline double grp_bcast_d(double v) {
   const long long u = __builtin_bit_cast(long long, v);
@@ -604,193 +567,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* ome
       if (run) spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
     }
     cur = nxt;
-  }
-  block_partials<SP, 2>(v, partials);
-}
-
-// Matrix data of one padded row for its lane group: per lane, entries ch * SP + s.
-template <int SP>
-struct RwRow {
-  static constexpr int NCH = (kRowWidth + SP - 1) / SP;
-  int c[NCH];
-  double a[NCH], b[NCH];
-};
-
-template <int SP>
-__device__ inline void rw_load_row(const RowPadView& A, int row, int s, RwRow<SP>& R) {
-#pragma unroll
-  for (int ch = 0; ch < RwRow<SP>::NCH; ++ch) {
-    const int k = ch * SP + s;
-    R.c[ch] = row;
-    R.a[ch] = 0.0;
-    R.b[ch] = 0.0;
-    if (k < kRowWidth) {
-      R.c[ch] = A.col[(size_t)row * kRowWidth + k];
-      R.a[ch] = A.a[(size_t)row * kRowWidth + k];
-      if (k < A.wb) R.b[ch] = A.b[(size_t)row * A.wb + k];
-    }
-  }
-}
-
-// Row-group SpMV over the padded row pack: as k_spmv_rg, with every address a function of the
-// row index, so a tile's matrix loads are issued together and the next tile's are in flight
-// while the current tile gathers; full 32-entry rows run without a branch. Padding entries
-// point at the row itself (x already in cache) with value 0.
-template <class T, int SP, int MODE>
-__global__ __launch_bounds__(kBlock) void k_spmv_rw(RowPadView A, const double* omega,
-                                                    const typename Vec2<T>::type* __restrict__ x,
-                                                    typename Vec2<T>::type* __restrict__ y,
-                                                    const typename Vec2<T>::type* __restrict__ b,
-                                                    typename Vec2<T>::type* __restrict__ ym,
-                                                    typename Vec2<T>::type* __restrict__ sm,
-                                                    const SysScalars* __restrict__ sc,
-                                                    const int* __restrict__ nrun,
-                                                    double* partials) {
-  static_assert(SP >= 4 && kBlock % SP == 0, "row groups of 4..64 lanes");
-  using T2 = typename Vec2<T>::type;
-  constexpr int NCH = RwRow<SP>::NCH;
-  constexpr int STEPS = SP < kRowWidth ? SP : kRowWidth;
-  if (*nrun == 0) return;
-  const long long N = (long long)A.nb * SP;
-  const long long ntiles = (N + kBlock - 1) / kBlock;
-  double v[2] = {0.0, 0.0};
-  TileWalk tw(ntiles);
-  const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP
-  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
-  const double w = omega[s];
-  // N is a multiple of SP: whole groups are in or out of range together
-  auto row_of = [&](long long tile) {
-    const long long t = tile * kBlock + threadIdx.x;
-    return t < N ? int(t / SP) : -1;
-  };
-  RwRow<SP> cur, nxt;
-  long long tile = tw.first;
-  if (tile < tw.last) {
-    const int r0 = row_of(tile);
-    if (r0 >= 0) rw_load_row<SP>(A, r0, s, cur);
-  }
-  for (; tile < tw.last; tile += tw.step) {
-    const int row = row_of(tile);
-    const long long ntile = tile + tw.step;
-    const int nrow = ntile < tw.last ? row_of(ntile) : -1;
-    if (nrow >= 0) rw_load_row<SP>(A, nrow, s, nxt);
-    if (row >= 0) {
-      const long long t = (long long)row * SP + s;
-      const T2 xs = x[t];
-      const double dA = A.da[row], dB = A.db[row] * w;
-      double yr = dA * double(xs.x) - dB * double(xs.y);
-      double yi = dA * double(xs.y) + dB * double(xs.x);
-      // per chunk: issue all gathers (only the column is broadcast first), then broadcast the
-      // values and accumulate; the scheduling barriers keep one chunk's gathers in flight at a
-      // time instead of letting every broadcast be hoisted into registers
-      const char* xb = reinterpret_cast<const char*>(x);
-      static_for<0, NCH>([&](auto CH) {
-        T2 xv[STEPS];
-        const int c = cur.c[CH];
-        static_for<0, STEPS>([&](auto J) {
-          const int cj = grp_bcast<SP, J>(c);
-          // 32-bit byte offsets (the batch of x is below 4 GiB, checked on the host)
-          const unsigned off = (unsigned)((run ? cj : 0) * SP + s) * (unsigned)sizeof(T2);
-          xv[J] = *reinterpret_cast<const T2*>(xb + off);
-        });
-        const double a = cur.a[CH], bb = cur.b[CH];
-        static_for<0, STEPS>([&](auto J) {
-          // broadcast the values only once the gathered x is there (see grp_bcast_after)
-          const int dep = __builtin_bit_cast(int, float(xv[J].x));
-          const double aj = grp_bcast_d_after<SP, J>(a, dep);
-          const double bj = grp_bcast_d_after<SP, J>(bb, dep) * w;
-          const double xr = double(xv[J].x), xm = double(xv[J].y);
-          yr += aj * xr - bj * xm;
-          yi += aj * xm + bj * xr;
-        });
-      });
-      if (run) spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
-    }
-    if (nrow >= 0) cur = nxt;
-  }
-  block_partials<SP, 2>(v, partials);
-}
-
-// The same product with the gathers landing in LDS (global_load_lds_dwordx4, FP64 vectors):
-// a gather in flight holds no registers, so a wave keeps G of them outstanding per batch at
-// full occupancy. Wave w owns LDS slots [w][0..G)[64]; lane l's value of step j lands in
-// slot [w][j][l] and only that lane reads it back, after the wave's own vmcnt drains, so no
-// workgroup barrier is involved.
-template <int SP, int MODE, int G>
-__global__ __launch_bounds__(kBlock) void k_spmv_rwl(RowPadView A, const double* omega,
-                                                     const double2* __restrict__ x,
-                                                     double2* __restrict__ y,
-                                                     const double2* __restrict__ b,
-                                                     double2* __restrict__ ym,
-                                                     double2* __restrict__ sm,
-                                                     const SysScalars* __restrict__ sc,
-                                                     const int* __restrict__ nrun,
-                                                     double* partials) {
-  static_assert(SP >= 4 && kBlock % SP == 0, "row groups of 4..64 lanes");
-  constexpr int NCH = RwRow<SP>::NCH;
-  constexpr int STEPS = SP < kRowWidth ? SP : kRowWidth;
-  static_assert(STEPS % G == 0, "whole batches per chunk");
-  __shared__ double2 land[kBlock / 64][G][64];
-  if (*nrun == 0) return;
-  const long long N = (long long)A.nb * SP;
-  const long long ntiles = (N + kBlock - 1) / kBlock;
-  double v[2] = {0.0, 0.0};
-  TileWalk tw(ntiles);
-  const int s = int(threadIdx.x % SP);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
-  const double w = omega[s];
-  auto row_of = [&](long long tile) {
-    const long long t = tile * kBlock + threadIdx.x;
-    return t < N ? int(t / SP) : -1;
-  };
-  RwRow<SP> cur, nxt;
-  long long tile = tw.first;
-  if (tile < tw.last) {
-    const int r0 = row_of(tile);
-    if (r0 >= 0) rw_load_row<SP>(A, r0, s, cur);
-  }
-  for (; tile < tw.last; tile += tw.step) {
-    const int row = row_of(tile);
-    const long long ntile = tile + tw.step;
-    const int nrow = ntile < tw.last ? row_of(ntile) : -1;
-    if (nrow >= 0) rw_load_row<SP>(A, nrow, s, nxt);
-    if (row >= 0) {
-      const long long t = (long long)row * SP + s;
-      const double2 xs = x[t];
-      const double dA = A.da[row], dB = A.db[row] * w;
-      double yr = dA * xs.x - dB * xs.y;
-      double yi = dA * xs.y + dB * xs.x;
-      static_for<0, NCH>([&](auto CH) {
-        const int c = cur.c[CH];
-        const double a = cur.a[CH], bb = cur.b[CH];
-        static_for<0, STEPS / G>([&](auto B) {
-          static_for<0, G>([&](auto J) {
-            const int cj = grp_bcast<SP, B * G + J>(c);
-            // 32-bit byte offset (the batch of x is below 4 GiB, checked on the host)
-            const unsigned off = (unsigned)((run ? cj : 0) * SP + s) * 16u;
-            const char* src = reinterpret_cast<const char*>(x) + off;
-            __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void*)src,
-                (__attribute__((address_space(3))) void*)&land[wave][J][0], 16, 0, 0);
-          });
-          __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's landings are in LDS
-          static_for<0, G>([&](auto J) {
-            const double2 xv = land[wave][J][lane];
-            // broadcast the values only after the landing is read back (grp_bcast_after), so
-            // that the compiler does not hold all of them in registers across the wait
-            const int dep = int(__builtin_bit_cast(long long, xv.x));
-            const double aj = grp_bcast_d_after<SP, B * G + J>(a, dep);
-            const double bj = grp_bcast_d_after<SP, B * G + J>(bb, dep) * w;
-            yr += aj * xv.x - bj * xv.y;
-            yi += aj * xv.y + bj * xv.x;
-          });
-          __builtin_amdgcn_wave_barrier();  // slots are reused by the next batch
-        });
-      });
-      if (run) spmv_store<double, MODE>(t, yr, yi, x, y, b, ym, sm, v);
-    }
-    if (nrow >= 0) cur = nxt;
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -1330,8 +1106,6 @@ struct mtcocr_handle {
   DevBuf offc, nc, offr, nr, colc, colr;
   DevBuf ac64, bc64, ar64, da64, db64;
   DevBuf rp, rb, pcol, pa, pb;  // row pack of the off-diagonal entries
-  DevBuf qcol, qa, qb;          // padded row pack (empty when a row is too long)
-  int wb = 0;
   // gradient
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0;
@@ -1349,10 +1123,8 @@ struct mtcocr_handle {
   // launch shapes (workgroup caps of the grid-stride kernels) per vector precision (FP64,
   // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
   int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
-  // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg),
-  // 2 padded row pack (k_spmv_rw), 3 / 4 padded row pack with the gathers landing in LDS in
-  // batches of 8 / 16 (k_spmv_rwl, FP64 vectors; 2 for FP32); 2-4 fall back to 1 when the
-  // padded pack is not built; 1-4 for batch widths 4..64
+  // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg,
+  // batch widths 4..64)
   int spmv_layout[2][7];
   static int sp_index(int sp) {
     int k = 0;
@@ -1371,12 +1143,12 @@ struct mtcocr_handle {
   mtcocr_handle() {
     set_caps(8192, 2048, 8192, 0);
     for (int p = 0; p < 2; ++p)
-      for (int k = 4; k < 7; ++k) spmv_layout[p][k] = 2;  // padded row pack from 16 systems up
+      for (int k = 4; k < 7; ++k) spmv_layout[p][k] = 1;  // row pack from 16 systems up
   }
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
-                           &db64, &rp, &rb, &pcol, &pa, &pb, &qcol, &qa, &qb, &gcol, &gtcol, &gval64, &gtval64,
+                           &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn,
                            &vx, &vb};
@@ -1458,18 +1230,6 @@ RowView row_view(mtcocr_handle* h) {
   return v;
 }
 
-RowPadView rowpad_view(mtcocr_handle* h) {
-  RowPadView v;
-  v.col = h->qcol.as<int>();
-  v.a = h->qa.as<double>();
-  v.b = h->qb.as<double>();
-  v.da = h->da64.as<double>();
-  v.db = h->db64.as<double>();
-  v.nb = h->nb;
-  v.wb = h->wb;
-  return v;
-}
-
 // SpMV launch in the layout chosen for this batch width (vector type TV, which the outer FP64
 // residual of the mixed solver uses with an FP32 engine).
 template <class TV, int SP, int MODE>
@@ -1479,27 +1239,6 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  typename Vec2<TV>::type* sm, const SysScalars* sc, const int* nrun,
                  double* part) {
   if constexpr (SP >= 4) {
-    // the padded kernels address x with 32-bit byte offsets
-    const bool small_x = (double)h->nb * SP * sizeof(typename Vec2<TV>::type) < 4294967296.0;
-    if (!small_x && layout >= 2) layout = 1;
-    if constexpr (std::is_same<TV, double>::value) {
-      if ((layout == 3 || layout == 4) && h->wb > 0) {
-        constexpr int STEPS = SP < kRowWidth ? SP : kRowWidth;
-        constexpr int G8 = STEPS % 8 ? STEPS : 8, G16 = STEPS % 16 ? G8 : 16;
-        if (layout == 3)
-          k_spmv_rwl<SP, MODE, G8><<<grid, kBlock, 0, st>>>(rowpad_view(h), omega, x, y, b, ym,
-                                                            sm, sc, nrun, part);
-        else
-          k_spmv_rwl<SP, MODE, G16><<<grid, kBlock, 0, st>>>(rowpad_view(h), omega, x, y, b, ym,
-                                                             sm, sc, nrun, part);
-        return;
-      }
-    }
-    if (layout >= 2 && h->wb > 0) {
-      k_spmv_rw<TV, SP, MODE><<<grid, kBlock, 0, st>>>(rowpad_view(h), omega, x, y, b, ym, sm, sc,
-                                                       nrun, part);
-      return;
-    }
     if (layout >= 1) {
       k_spmv_rg<TV, SP, MODE><<<grid, kBlock, 0, st>>>(row_view(h), omega, x, y, b, ym, sm, sc,
                                                        nrun, part);
@@ -1977,7 +1716,6 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
   h->nb = E.nb;
   h->nslices = E.nslices;
   h->nnz_c = E.nnz_complex;
-  h->wb = E.wb;
   h->nnz_r = E.nnz_real;
   h->ks_form = ks;
   if ((rc = upload(h, h->offc, E.offc, st)) || (rc = upload(h, h->nc, E.nc, st)) ||
@@ -1987,9 +1725,7 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
       (rc = upload(h, h->ar64, E.ar, st)) || (rc = upload(h, h->da64, E.diag_a, st)) ||
       (rc = upload(h, h->db64, E.diag_b, st)) || (rc = upload(h, h->rp, E.rp, st)) ||
       (rc = upload(h, h->rb, E.rb, st)) || (rc = upload(h, h->pcol, E.pcol, st)) ||
-      (rc = upload(h, h->pa, E.pa, st)) || (rc = upload(h, h->pb, E.pb, st)) ||
-      (rc = upload(h, h->qcol, E.qcol, st)) || (rc = upload(h, h->qa, E.qa, st)) ||
-      (rc = upload(h, h->qb, E.qb, st)))
+      (rc = upload(h, h->pa, E.pa, st)) || (rc = upload(h, h->pb, E.pb, st)))
     return rc;
   HIPC(hipStreamSynchronize(st));
   h->have_matrix = true;
@@ -2306,7 +2042,7 @@ int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
   if (!h) return MTCOCR_ERR_ARG;
   auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
   if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
-  if (spmv_layout < -1 || spmv_layout > 4) return MTCOCR_ERR_ARG;
+  if (spmv_layout < -1 || spmv_layout > 1) return MTCOCR_ERR_ARG;
   h->set_caps(cap_spmv, cap_vec, cap_node, spmv_layout);
   drop_graph(h);  // the captured iterations hold the old grids
   return 0;
@@ -2370,31 +2106,13 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         HIPC(hipGetLastError());
       }
       for (int fam = 0; fam < 3; ++fam) {
-        const int nlayouts =
-            (fam == 0 && SP >= 4)
-                ? (h->wb > 0 ? (std::is_same<T, double>::value ? 5 : 3) : 2)
-                : 1;
+        const int nlayouts = (fam == 0 && SP >= 4) ? 2 : 1;
         for (int layout = 0; layout < nlayouts; ++layout) {
           // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
           // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
           // twice that
           int per_cu = 0;
-          if (fam == 0 && layout >= 3) {
-            if constexpr (SP >= 4) {
-              constexpr int STEPS = SP < kRowWidth ? SP : kRowWidth;
-              constexpr int G8 = STEPS % 8 ? STEPS : 8, G16 = STEPS % 16 ? G8 : 16;
-              if (layout == 3)
-                HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(k_spmv_rwl<SP, 0, G8>), kBlock, 0));
-              else
-                HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(k_spmv_rwl<SP, 0, G16>), kBlock, 0));
-            }
-          } else if (fam == 0 && layout == 2) {
-            if constexpr (SP >= 4)
-              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, reinterpret_cast<const void*>(k_spmv_rw<T, SP, 0>), kBlock, 0));
-          } else if (fam == 0 && layout == 1) {
+          if (fam == 0 && layout == 1) {
             if constexpr (SP >= 4)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_rg<T, SP, 0>), kBlock, 0));
