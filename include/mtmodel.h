@@ -94,6 +94,22 @@ int mtmodel_assemble_ks(int nxn, int nyn, int nzn, const double* x, const double
  * b = curl-curl (no 1/mu), c = mass (no sigma). Used by tests and by the adjoint. */
 int mtmodel_element_matrices(double hx, double hy, double hz, double* b, double* c);
 
+/* Edge DOFs (complex interleaved, a0[2*nb]) of the 1-D normal field of polarization alpha, the
+ * values the right-hand side is built from: (A0(z_e) (alpha, 1 - alpha, 0) . t_e) h_e / 2 at the
+ * height of horizontal edges, 0 on vertical edges (Code/BuildMatrix/T_Brick.cpp:1361-1495
+ * evaluates the same normal field per element). Used by the adjoint: the total field is the
+ * solution plus these DOFs. */
+int mtmodel_normal_dofs(int nxn, int nyn, int nzn, const double* x, const double* y,
+                        const double* z, int alpha, int n1d, const double* z1d,
+                        const double* usin, const double* ucos, double* a0);
+
+/* Per cell c: out[2c], out[2c+1] = u_c^T M_c v_c (complex, unconjugated) with M_c the cell's
+ * 12x12 mass matrix (the sigma-derivative of the system matrix up to i*omega) and u, v
+ * complex interleaved edge vectors. No counterpart in the reference (it has no gradient); used
+ * by the adjoint sensitivity dJ/dsigma_c = 2 Re[-i*omega lambda_c^T M_c (x + a0)_c]. */
+int mtmodel_cell_mass_forms(int nxn, int nyn, int nzn, const double* x, const double* y,
+                            const double* z, const double* u, const double* v, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,10 @@ SIGNATURES = {
     "mtmodel_assemble_ks": (C.c_int, [C.c_int, C.c_int, C.c_int, f64p, f64p, f64p, f64p, f64p,
                                       i32p, i32p, f64p, f64p, f64p, f64p]),
     "mtmodel_element_matrices": (C.c_int, [C.c_double, C.c_double, C.c_double, f64p, f64p]),
+    "mtmodel_normal_dofs": (C.c_int, [C.c_int, C.c_int, C.c_int, f64p, f64p, f64p, C.c_int,
+                                      C.c_int, f64p, f64p, f64p, f64p]),
+    "mtmodel_cell_mass_forms": (C.c_int, [C.c_int, C.c_int, C.c_int, f64p, f64p, f64p, f64p,
+                                          f64p, f64p]),
     # mtcocr.h
     "mtcocr_error_string": (C.c_char_p, [C.c_int]),
     "mtcocr_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),

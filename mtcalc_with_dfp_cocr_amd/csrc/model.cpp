@@ -760,4 +760,63 @@ int mtmodel_assemble_ks(int nxn, int nyn, int nzn, const double* x, const double
   return 0;
 }
 
+int mtmodel_normal_dofs(int nxn, int nyn, int nzn, const double* x, const double* y,
+                        const double* z, int alpha, int n1d, const double* z1d,
+                        const double* usin, const double* ucos, double* a0) {
+  if (!grid_ok(nxn, nyn, nzn) || !x || !y || !z || !z1d || !usin || !ucos || !a0 || n1d < 2)
+    return MTMODEL_ERR_ARG;
+  Grid g{nxn, nyn, nzn};
+  EdgeGeom eg;
+  build_edge_geom(g, eg);
+  const long long nb = g.edges();
+  const double alpha_d = double(alpha);
+#pragma omp parallel for schedule(static)
+  for (long long e = 0; e < nb; ++e) {
+    const int d = eg.dir[e];
+    double re = 0.0, im = 0.0;
+    if (d != 2) {
+      // the DOF the right-hand side uses: (u . t) h / 2 at the edge's height, u = A0 (alpha,
+      // 1 - alpha, 0) (see assemble_impl / mtmodel_assemble_rhs)
+      const double zm = 0.5 * (z[eg.iz[e]] + z[eg.iz[e]]);
+      const double h = d == 0 ? x[eg.ix[e] + 1] - x[eg.ix[e]] : y[eg.iy[e] + 1] - y[eg.iy[e]];
+      const double w = d == 0 ? alpha_d : 1.0 - alpha_d;
+      re = spline(zm, n1d, z1d, usin) * w * (h * 0.5);
+      im = spline(zm, n1d, z1d, ucos) * w * (h * 0.5);
+    }
+    a0[2 * e] = re;
+    a0[2 * e + 1] = im;
+  }
+  return 0;
+}
+
+int mtmodel_cell_mass_forms(int nxn, int nyn, int nzn, const double* x, const double* y,
+                            const double* z, const double* u, const double* v, double* out) {
+  if (!grid_ok(nxn, nyn, nzn) || !x || !y || !z || !u || !v || !out) return MTMODEL_ERR_ARG;
+  Grid g{nxn, nyn, nzn};
+  EdgeNumbering num(g);
+  const long long ncx = g.nxn - 1, ncy = g.nyn - 1, ncz = g.nzn - 1;
+#pragma omp parallel for schedule(static)
+  for (long long cid = 0; cid < ncx * ncy * ncz; ++cid) {
+    const long long cx = cid % ncx, cy = (cid / ncx) % ncy, cz = cid / (ncx * ncy);
+    double b[12][12], cm[12][12];
+    element_matrices(x[cx + 1] - x[cx], y[cy + 1] - y[cy], z[cz + 1] - z[cz], b, cm);
+    long long ce[12];
+    num.cell_edges(cx, cy, cz, ce);
+    double sr = 0.0, si = 0.0;  // sum_jk u_j M_jk v_k, complex, unconjugated
+    for (int j = 0; j < 12; ++j) {
+      const double ur = u[2 * ce[j]], ui = u[2 * ce[j] + 1];
+      double tr = 0.0, ti = 0.0;
+      for (int k = 0; k < 12; ++k) {
+        tr += cm[j][k] * v[2 * ce[k]];
+        ti += cm[j][k] * v[2 * ce[k] + 1];
+      }
+      sr += ur * tr - ui * ti;
+      si += ur * ti + ui * tr;
+    }
+    out[2 * cid] = sr;
+    out[2 * cid + 1] = si;
+  }
+  return 0;
+}
+
 }  // extern "C"

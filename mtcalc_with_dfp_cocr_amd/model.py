@@ -82,6 +82,11 @@ class Model:
         return len(self.x) - 1, len(self.y) - 1, len(self.z) - 1
 
     @property
+    def n_cells(self) -> int:
+        a, b, c = self.shape_cells
+        return a * b * c
+
+    @property
     def n_edges(self) -> int:
         return int(lib().mtmodel_num_edges(*self.shape_nodes))
 

@@ -77,3 +77,38 @@ def identity_system(M):
                       np.arange(0, 2 * nb + 1, 2, dtype=np.int32), np.zeros(1, np.int32), di,
                       np.zeros(0), pr)
     return m, ma, s
+
+
+class DirectKSSolver:
+    """Sparse LU solves of A(omega) = K + i*omega*S assembled for a conductivity (the exact
+    reference for gradient checks; scipy's SuperLU, no iteration)."""
+
+    def __init__(self, M, m):
+        self.M, self.m = M, m
+        self._sigma = None
+
+    def set_sigma(self, sigma):
+        import scipy.sparse as sp
+
+        from mtcalc_with_dfp_cocr_amd.adjoint import with_sigma
+
+        if self._sigma is not None and np.array_equal(sigma, self._sigma):
+            return
+        p, kd, kg, sd, sg = self.M.assemble_ks(with_sigma(self.m, sigma))
+        rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+        full = lambda d, g: (sp.csr_matrix((g, (rows, p.jg)), shape=(p.nb, p.nb))
+                             + sp.csr_matrix((g, (p.jg, rows)), shape=(p.nb, p.nb))
+                             + sp.diags(d))
+        self.K, self.S = full(kd, kg), full(sd, sg)
+        self._lu = {}
+        self._sigma = np.array(sigma, copy=True)
+
+    def solve(self, omegas, rhs):
+        import scipy.sparse.linalg as spl
+
+        out = np.empty_like(rhs)
+        for k, w in enumerate(omegas):
+            if w not in self._lu:
+                self._lu[w] = spl.splu((self.K + 1j * w * self.S).tocsc())
+            out[k] = self._lu[w].solve(rhs[k])
+        return out

@@ -350,3 +350,39 @@ def test_mesh_order_parity(O, M, c1_system, c1_mesh, oracle_c1, block):
         assert rel(xs, xo) <= 1e-5
         with pytest.raises(RuntimeError):
             B.set_mesh_order(c1_mesh, block)  # after the matrix: refused
+
+
+def test_adjoint_gradient_gpu(M, small_model):
+    """Adjoint gradient with the GPU batched solver (forward and adjoint systems of 2 frequencies
+    x 2 polarizations each in one batch) against the same gradient from exact sparse LU
+    solves."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from helpers import DirectKSSolver
+
+    from mtcalc_with_dfp_cocr_amd.adjoint import GpuBatchSolver, ImpedanceMisfit
+
+    say("adjoint gradient on the GPU")
+    m = small_model
+    rng = np.random.default_rng(21)
+    rec = np.stack([rng.uniform(m.x[3], m.x[-4], 5), rng.uniform(m.y[3], m.y[-4], 5),
+                    np.zeros(5)], 1)
+    freqs = [0.1, 3.0]
+    ref = ImpedanceMisfit(m, freqs, rec, solver=DirectKSSolver(M, m))
+    earth = m.sigma > 1e-6
+    sig_true = m.sigma.copy()
+    sig_true[earth] *= np.exp(0.3 * rng.standard_normal(int(earth.sum())))
+    _, zobs, _ = ref.responses(sig_true)
+    ref.z_obs = zobs
+    st = ref.evaluate(m.sigma)
+    g_ref = ref.gradient(st)
+    gpu = GpuBatchSolver(m, eps=1e-12)
+    try:
+        prob = ImpedanceMisfit(m, freqs, rec, z_obs=zobs, solver=gpu)
+        st2 = prob.evaluate(m.sigma)
+        g = prob.gradient(st2)
+    finally:
+        gpu.close()
+    say(f"J gpu {st2.J:.12e} direct {st.J:.12e}")
+    assert st2.J == pytest.approx(st.J, rel=1e-8)
+    assert np.linalg.norm(g - g_ref) <= 1e-6 * np.linalg.norm(g_ref)
