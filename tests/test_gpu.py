@@ -385,4 +385,9 @@ def test_adjoint_gradient_gpu(M, small_model):
         gpu.close()
     say(f"J gpu {st2.J:.12e} direct {st.J:.12e}")
     assert st2.J == pytest.approx(st.J, rel=1e-8)
-    assert np.linalg.norm(g - g_ref) <= 1e-6 * np.linalg.norm(g_ref)
+    # in the inversion variable log(sigma): dJ/dlog(sigma) = sigma dJ/dsigma (dJ/dsigma itself
+    # is dominated by the 1e-8 S/m air cells, where it is large and irrelevant)
+    gl, gl_ref = g * m.sigma, g_ref * m.sigma
+    err = np.linalg.norm(gl - gl_ref) / np.linalg.norm(gl_ref)
+    say(f"gradient rel err (log sigma) {err:.3e}")
+    assert err <= 1e-6
