@@ -92,8 +92,14 @@ class GpuBatchSolver:
         self._sigma = np.array(sigma, copy=True)
 
     def solve(self, omegas: np.ndarray, rhs: np.ndarray) -> np.ndarray:
-        """rhs: (nsys, nb) complex. Returns (nsys, nb) complex."""
+        """rhs: (nsys, nb) complex. Returns (nsys, nb) complex. Each right-hand side is solved
+        at unit norm and the solution scaled back: the stop test is relative, but the COCR
+        breakdown test on (a, w) is absolute (1e-40, rci.cpp:38) and would fire near a tight
+        tolerance on a small right-hand side."""
         out = np.empty_like(rhs)
+        scale = np.linalg.norm(rhs, axis=1)
+        scale[scale == 0] = 1.0
+        rhs = rhs / scale[:, None]
         for k0 in range(0, len(omegas), 64):
             k1 = min(len(omegas), k0 + 64)
             prs = np.stack([M.to_interleaved(b) for b in rhs[k0:k1]])
@@ -104,7 +110,7 @@ class GpuBatchSolver:
             if bad:
                 raise RuntimeError(f"batched solve did not converge: {bad[0]}")
             out[k0:k1] = x[:, 0::2] + 1j * x[:, 1::2]
-        return out
+        return out * scale[:, None]
 
     def close(self):
         self.S.close()
