@@ -138,6 +138,36 @@ def measure_traffic(args, sp):
     return res
 
 
+def impedance_agreement(M, m, cases, x_batched, x_single, x_mixed):
+    """Largest relative difference of Zxy / Zyx at a 5x5 receiver grid (200 m spacing, centred)
+    per frequency: batched FP64 vs one-at-a-time FP64, and mixed vs batched FP64."""
+    from mtcalc_with_dfp_cocr_amd.receivers import ReceiverOperator, impedance
+
+    g = np.arange(-400.0, 401.0, 200.0)
+    rx, ry = np.meshgrid(g, g, indexing="ij")
+    R = ReceiverOperator(m, np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1))
+    freqs = sorted({c[0] for c in cases})
+    idx = {c: k for k, c in enumerate(cases)}
+    to_c = lambda v: v[0::2] + 1j * v[1::2]
+
+    def Z(xs, f):
+        Ex, Hx = R.fields(to_c(xs[idx[(f, 1)]]), f, 1)
+        Ey, Hy = R.fields(to_c(xs[idx[(f, 0)]]), f, 0)
+        return impedance(Ex, Hx, Ey, Hy)
+
+    def rel(a, b):
+        return float(max(np.max(np.abs(a[:, i, j] - b[:, i, j]) / np.abs(b[:, i, j]))
+                         for i, j in ((0, 1), (1, 0))))
+
+    out = {"receivers": "5x5, 200 m spacing", "freqs": [float(f) for f in freqs]}
+    have = [f for f in freqs if idx[(f, 1)] in x_single and idx[(f, 0)] in x_single]
+    if have:
+        out["batched_vs_single_fp64"] = [rel(Z(x_batched, f), Z(x_single, f)) for f in have]
+    if x_mixed is not None:
+        out["mixed_vs_fp64"] = [rel(Z(x_mixed, f), Z(x_batched, f)) for f in freqs]
+    return out
+
+
 def timed(S, prec, args):
     sync()
     t0 = time.perf_counter()
@@ -169,6 +199,8 @@ def main():
                     help="pencil width of the device's locality order (0: the file order)")
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="launches per candidate shape in the autotuner")
+    ap.add_argument("--adjoint-cells", type=int, default=4,
+                    help="cells of the config-4 adjoint vs finite-difference check (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure SpMV HBM traffic")
     ap.add_argument("--kernels-only", action="store_true",
@@ -234,10 +266,12 @@ def main():
 
     # ---- mixed precision, batched ---------------------------------------------------------
     mixed = None
+    x_mixed = None
     if not args.no_mixed:
         S.set_rhs(prs, om)
         dt, rc, st = timed(S, "mixed", args)
         xm = S.get_solution()
+        x_mixed = xm
         mixed = {"s_per_system": dt / nsys, "ok": all(q["req"] == 0 for q in st),
                  "true_rel_res_max": max(q["true_rel_res"] for q in st),
                  "inner_iters": [q["iter"] for q in st], "passes": [q["outer"] for q in st],
@@ -250,10 +284,12 @@ def main():
     # ---- comparison: plain FP64 GPU solver, one system at a time ----------------------------
     n64 = nsys if args.fp64_systems < 0 else min(nsys, args.fp64_systems)
     fp64 = []
+    x_single = {}
     for k in range(n64):
         S.set_rhs(prs[k:k + 1], om[k:k + 1])
         dt, rc, st = timed(S, "fp64", args)
         x64 = S.get_solution()[0]
+        x_single[k] = x64
         d = np.linalg.norm(x64 - x_batched[k]) / np.linalg.norm(x64)
         fp64.append({"f": float(cases[k][0]), "pol": "X" if cases[k][1] else "Y", "s": dt,
                      "iter": st[0]["iter"], "batched_iter": stats[k]["iter"], "req": st[0]["req"],
@@ -272,6 +308,27 @@ def main():
     log(f"spmv fp64 x{nsys}: {spmv_ms:.3f} ms {ach:.0f} GB/s; fp32-vector x{nsys}: "
         f"{spmv32_ms:.3f} ms {ach32:.0f} GB/s; fp64 x1: {spmv64_ms:.3f} ms {ach64:.0f} GB/s")
     S.close()
+
+    # ---- impedance agreement at a 5x5 surface grid (200 m spacing) ---------------------------
+    zcmp = impedance_agreement(M, m, cases, x_batched, x_single, x_mixed)
+    log(f"impedance agreement: {zcmp}")
+
+    # ---- adjoint gradient (config 4) against central differences ----------------------------
+    adj = None
+    if args.adjoint_cells > 0:
+        from mtcalc_with_dfp_cocr_amd import adjoint as AD
+
+        t0 = time.perf_counter()
+        import io
+        from contextlib import redirect_stdout
+
+        buf = io.StringIO()
+        with redirect_stdout(buf):
+            AD.main(["--fd-cells", str(args.adjoint_cells)])
+        adj = json.loads(buf.getvalue().strip().splitlines()[-1])
+        adj = {k: adj[k] for k in ("workload", "J", "eps", "h", "rel_err_norm", "rel_err",
+                                   "seconds", "timings")}
+        log(f"adjoint (config 4): {adj} ({time.perf_counter() - t0:.1f}s)")
 
     # ---- CPU baseline -------------------------------------------------------------------
     cpu = None
@@ -329,6 +386,8 @@ def main():
         "spmv_fp64_single": {"achieved_gbs": ach64, "avg_ms": spmv64_ms,
                              "algorithmic_bytes": spmv64_bytes, "frac": ach64 / HBM_PEAK_GBS},
         "fp64_systems": fp64,
+        "impedance_agreement": zcmp,
+        "adjoint_config4": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -391,3 +391,45 @@ def test_adjoint_gradient_gpu(M, small_model):
     err = np.linalg.norm(gl - gl_ref) / np.linalg.norm(gl_ref)
     say(f"gradient rel err (log sigma) {err:.3e}")
     assert err <= 1e-6
+
+
+def test_impedance_parity_with_oracle(O, M, c1_model, c1_mesh):
+    """Z (and rho_a, phase) at the 25 config-1 receivers from the GPU solutions of both
+    polarizations against Z from the oracle's solutions: within 1e-6 relative."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+    from mtcalc_with_dfp_cocr_amd.receivers import (ReceiverOperator, apparent_resistivity,
+                                                    impedance)
+
+    say("impedance parity, config 1, 1 Hz")
+    nu = 1.0
+    R = ReceiverOperator(c1_model, c1_model.receivers)
+    sols = {}
+    with Solver(0) as S:
+        S.set_mesh_order(c1_mesh, 16)
+        for alpha in (1, 0):
+            s = M.assemble(c1_model, nu, alpha)
+            if alpha == 1:
+                S.set_matrix(s)
+                S.set_gradient(c1_mesh)
+            xg, st = S.solve(s.pr, eps=1e-8, precision="fp64")
+            xo, so, _ = O.pcocr(s, c1_mesh, eps=1e-8)
+            assert st["req"] == 0 and so["req"] == 0
+            sols[alpha] = (M.to_complex(xg), M.to_complex(xo))
+    Zs = []
+    for k in (0, 1):
+        Ex, Hx = R.fields(sols[1][k], nu, 1)
+        Ey, Hy = R.fields(sols[0][k], nu, 0)
+        Zs.append(impedance(Ex, Hx, Ey, Hy))
+    Zg, Zo = Zs
+    off = [(0, 1), (1, 0)]
+    for i, j in off:
+        err = np.abs(Zg[:, i, j] - Zo[:, i, j]) / np.abs(Zo[:, i, j])
+        say(f"Z{'xy' if i == 0 else 'yx'} max rel diff {err.max():.2e}")
+        assert err.max() <= 1e-6
+    rg, pg = apparent_resistivity(Zg, nu)
+    ro, po = apparent_resistivity(Zo, nu)
+    for i, j in off:
+        np.testing.assert_allclose(rg[:, i, j], ro[:, i, j], rtol=2e-6)
+        np.testing.assert_allclose(pg[:, i, j], po[:, i, j], atol=1e-4)
