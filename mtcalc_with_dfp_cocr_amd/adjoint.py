@@ -312,10 +312,13 @@ def main(argv=None):
     t2 = time.perf_counter()
     grad = prob.gradient(st) * sigma
     t3 = time.perf_counter()
-    # FD cells: seeded among earth cells under the receiver area, within 600 m of the surface
+    # FD cells: seeded among earth cells under the receiver area, within 600 m of the surface,
+    # whose sensitivity is above the finite differences' noise floor (|dJ/dlog sigma| >= 1e-3 of
+    # the largest: the solves stop at eps, so J carries ~eps J of noise, ~1e-6 after / 2h)
     xc, yc, zc = m.cell_centres()
+    big = np.abs(grad) >= 1e-3 * np.abs(grad).max()
     cand = np.flatnonzero((zc < 0) & (zc > -600) & (np.abs(xc) < np.abs(rec[:, 0]).max())
-                          & (np.abs(yc) < np.abs(rec[:, 1]).max()))
+                          & (np.abs(yc) < np.abs(rec[:, 1]).max()) & big)
     cells = np.random.default_rng(a.seed + 7).choice(cand, size=min(a.fd_cells, len(cand)),
                                                      replace=False)
     fd = _fd_check(prob, sigma, cells, a.h)
