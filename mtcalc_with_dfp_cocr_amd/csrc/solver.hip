@@ -47,7 +47,7 @@ constexpr int kScalarThreads = 1024;
 constexpr int kXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 enum : int { ST_RUNNING = 1, ST_INACTIVE = 2 };
-enum : int { MRS_SKIP = 0, MRS_COPY = 1, MRS_BLEND = 2 };
+enum : int { MRS_SKIP = 0, MRS_COPY = 1, MRS_BLEND = 2, MRS_NONE = 3 };
 
 template <class T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
@@ -61,7 +61,8 @@ struct SysScalars {
   double scale, prev_rel, b_norm, best_rel;
   int mode, status, iter, total_iter;
   int outer_status, outer, last_improve, stall_window;
-  int fallback, fallback_iter, rollback, pad1;
+  int fallback, fallback_iter, rollback, halt;
+  int last, pad1, pad2, pad3;  // last: the iteration whose x is the answer (buffer last % 2)
 };
 
 template <class T>
@@ -688,78 +689,79 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
   block_partials<SP, 2>(v, partials);
 }
 
-// Stage 6 tail + MRS first pass: x += alpha p, r -= alpha a, u = r - sm, partials (sm,u),(u,u)
+// One pass per iteration k over the edge vectors (pcocr_rci.cpp:139-154, MRS.cpp:85-153):
+//   1. the MRS step of iteration k - 1, whose weight s_mrs_stop computed from the previous
+//      pass: ym += w (x - ym), sm += w (r - sm) (or a copy, or nothing), partial ||sm||^2 for
+//      the deferred stop test of k - 1;
+//   2. iteration k: x += alpha p (into the other x buffer, so that x of k - 1 survives if k - 1
+//      turns out to be the last iteration), r -= alpha a, s -= alpha w, and the partials
+//      (sm, u), (u, u) of u = r - sm for the MRS weight of k.
+// Step 2 is skipped by a system that halts in iteration k. Deferring the MRS step saves
+// re-reading x and r: 8 vector reads and 5 writes instead of 11 and 5 in two passes.
 template <class T, int SP>
-__global__ __launch_bounds__(kBlock) void k_xr_mrs1(
-    typename Vec2<T>::type* __restrict__ x, const typename Vec2<T>::type* __restrict__ p,
-    typename Vec2<T>::type* __restrict__ r, const typename Vec2<T>::type* __restrict__ a,
-    const typename Vec2<T>::type* __restrict__ sm, const SysScalars* __restrict__ sc,
+__global__ __launch_bounds__(kBlock) void k_xr_fused(
+    const typename Vec2<T>::type* __restrict__ xin, typename Vec2<T>::type* __restrict__ xout,
+    const typename Vec2<T>::type* __restrict__ p, typename Vec2<T>::type* __restrict__ r,
+    const typename Vec2<T>::type* __restrict__ a, typename Vec2<T>::type* __restrict__ s_vec,
+    const typename Vec2<T>::type* __restrict__ w, typename Vec2<T>::type* __restrict__ ym,
+    typename Vec2<T>::type* __restrict__ sm, const SysScalars* __restrict__ sc,
     const int* __restrict__ nrun, double* partials, int nb) {
   using T2 = typename Vec2<T>::type;
   if (*nrun == 0) return;
   const long long N = (long long)nb * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
-  double v[2] = {0.0, 0.0};
+  double v[3] = {0.0, 0.0, 0.0};
   TileWalk tw(ntiles);
   for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
     const int s = int(t % SP);
     if (sc[s].status != ST_RUNNING) continue;
-    const double ar = sc[s].al_re, ai = sc[s].al_im;
-    const T2 xn = caxpy(x[t], p[t], ar, ai);
-    const T2 rn = caxpy(r[t], a[t], -ar, -ai);
-    const T2 smv = sm[t];
-    const T ur = rn.x - smv.x, ui = rn.y - smv.y;
-    x[t] = xn;
-    r[t] = rn;
-    v[0] += double(smv.x) * double(ur) + double(smv.y) * double(ui);
-    v[1] += double(ur) * double(ur) + double(ui) * double(ui);
-  }
-  block_partials<SP, 2>(v, partials);
-}
-
-// MRS second pass (MRS.cpp:113-148) and stage 7's s -= alpha w; partial ||sm||^2.
-template <class T, int SP>
-__global__ __launch_bounds__(kBlock) void k_mrs2(
-    const typename Vec2<T>::type* __restrict__ x, const typename Vec2<T>::type* __restrict__ r,
-    typename Vec2<T>::type* __restrict__ ym, typename Vec2<T>::type* __restrict__ sm,
-    typename Vec2<T>::type* __restrict__ s_vec, const typename Vec2<T>::type* __restrict__ w,
-    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, double* partials, int nb) {
-  using T2 = typename Vec2<T>::type;
-  if (*nrun == 0) return;
-  const long long N = (long long)nb * SP;
-  const long long ntiles = (N + kBlock - 1) / kBlock;
-  double v[1] = {0.0};
-  TileWalk tw(ntiles);
-  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
-    const long long t = tile * kBlock + threadIdx.x;
-    if (t >= N) continue;
-    const int s = int(t % SP);
-    if (sc[s].status != ST_RUNNING) continue;
+    const T2 xv = xin[t], rv = r[t];
+    T2 smv = sm[t];
     const int mode = sc[s].mode;
     if (mode == MRS_BLEND) {
       const T wm = T(sc[s].w);
-      const T2 xv = x[t], rv = r[t];
-      T2 yv = ym[t], sv = sm[t];
-      const T ur = rv.x - sv.x, ui = rv.y - sv.y;
+      T2 yv = ym[t];
+      const T ur = rv.x - smv.x, ui = rv.y - smv.y;
       yv.x += wm * (xv.x - yv.x);
       yv.y += wm * (xv.y - yv.y);
-      sv.x += wm * ur;
-      sv.y += wm * ui;
+      smv.x += wm * ur;
+      smv.y += wm * ui;
       ym[t] = yv;
-      sm[t] = sv;
-      v[0] += double(sv.x) * double(sv.x) + double(sv.y) * double(sv.y);
+      sm[t] = smv;
+      v[2] += double(smv.x) * double(smv.x) + double(smv.y) * double(smv.y);
     } else if (mode == MRS_COPY) {
-      const T2 rv = r[t];
-      ym[t] = x[t];
-      sm[t] = rv;
-      v[0] += double(rv.x) * double(rv.x) + double(rv.y) * double(rv.y);
+      ym[t] = xv;
+      smv = rv;
+      sm[t] = smv;
+      v[2] += double(rv.x) * double(rv.x) + double(rv.y) * double(rv.y);
     }
+    if (sc[s].halt) continue;
     const double ar = sc[s].al_re, ai = sc[s].al_im;
+    const T2 xn = caxpy(xv, p[t], ar, ai);
+    const T2 rn = caxpy(rv, a[t], -ar, -ai);
     s_vec[t] = caxpy(s_vec[t], w[t], -ar, -ai);
+    xout[t] = xn;
+    r[t] = rn;
+    const T ur = rn.x - smv.x, ui = rn.y - smv.y;
+    v[0] += double(smv.x) * double(ur) + double(smv.y) * double(ui);
+    v[1] += double(ur) * double(ur) + double(ui) * double(ui);
   }
-  block_partials<SP, 1>(v, partials);
+  block_partials<SP, 3>(v, partials);
+}
+
+// x of each system's last iteration into x0: iteration k writes buffer k % 2
+template <class T, int SP>
+__global__ void k_final_x(typename Vec2<T>::type* __restrict__ x0,
+                          const typename Vec2<T>::type* __restrict__ x1,
+                          const SysScalars* __restrict__ sc, int nb) {
+  const long long N = (long long)nb * SP;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = int(t % SP);
+    if (sc[s].status != ST_INACTIVE && (sc[s].last & 1)) x0[t] = x1[t];
+  }
 }
 
 // ---- scalar kernels (one block) ----------------------------------------------------------
@@ -782,6 +784,9 @@ __global__ __launch_bounds__(kScalarThreads) void s_prologue(const double* part,
     sc[s].last_improve = 0;
     sc[s].be_re = sc[s].be_im = 0.0;
     sc[s].iter = 0;
+    sc[s].mode = MRS_NONE;  // no MRS step pending before iteration 1
+    sc[s].halt = 0;
+    sc[s].last = 0;
     if (rn < 1e-30) sc[s].status = MTCOCR_X0_NULL;
   }
   __syncthreads();
@@ -795,12 +800,13 @@ __global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, in
   __shared__ double tot[SP * 2];
   reduce_partials<SP * 2>(part, nblk, tot);
   const int s = threadIdx.x;
-  if (s < SP && sc[s].status == ST_RUNNING) {
+  if (s < SP && sc[s].status == ST_RUNNING && sc[s].halt == 0) {
     const double awr = tot[s * 2], awi = tot[s * 2 + 1];
     sc[s].aw_re = awr;
     sc[s].aw_im = awi;
     if (fabs(awr) < 1e-40 && fabs(awi) < 1e-40) {
-      sc[s].status = MTCOCR_DIV_BY_ZERO;
+      // takes effect after the previous iteration's deferred stop test (s_mrs_stop)
+      sc[s].halt = MTCOCR_DIV_BY_ZERO;
     } else {
       double qr, qi;
       cdiv(sc[s].zs_re, sc[s].zs_im, awr, awi, qr, qi);
@@ -812,53 +818,61 @@ __global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, in
   if (threadIdx.x == 0) count_running(sc, SP, nrun);
 }
 
-// after k_xr_mrs1: MRS weight w = -(s,u)/(u,u) clamped (MRS.cpp:113-121)
+// After k_xr_fused of iteration k: first the stop test of iteration k - 1, whose MRS step the
+// fused pass has just completed (rci.cpp:66-77 on the smoothed residual ||sm||), then a halt
+// decided during iteration k (division by zero, iteration limit), then the MRS weight of
+// iteration k, w = -(sm,u)/(u,u) clamped (MRS.cpp:113-121), applied by the next fused pass.
 template <int SP>
-__global__ __launch_bounds__(kScalarThreads) void s_mrs(const double* part, int nblk,
-                                                        SysScalars* sc) {
-  __shared__ double tot[SP * 2];
-  reduce_partials<SP * 2>(part, nblk, tot);
+__global__ __launch_bounds__(kScalarThreads) void s_mrs_stop(const double* part, int nblk,
+                                                             SysScalars* sc, int* nrun) {
+  __shared__ double tot[SP * 3];
+  reduce_partials<SP * 3>(part, nblk, tot);
   const int s = threadIdx.x;
   if (s < SP && sc[s].status == ST_RUNNING) {
-    const double su = tot[s * 2], uu = tot[s * 2 + 1];
-    double w = (fabs(uu) < 1e-100) ? 0.0 : -su / uu;
-    int mode;
-    if (w < 0) {
-      w = 0;
-      mode = MRS_SKIP;
-    } else if (w > 1) {
-      w = 1;
-      mode = MRS_COPY;
-    } else {
-      mode = MRS_BLEND;
+    const double su = tot[s * 3], uu = tot[s * 3 + 1], ss = tot[s * 3 + 2];
+    const int k = sc[s].iter;  // this iteration; the pending MRS step belongs to k - 1
+    bool stopped = false;
+    const int prev = sc[s].mode;
+    if (prev != MRS_NONE) {
+      if (prev == MRS_BLEND) sc[s].res = sqrt(ss);
+      else if (prev == MRS_COPY) sc[s].res = ss > 0.0 ? sqrt(ss) : 0.0;
+      sc[s].rel = sc[s].res / sc[s].r_old;
+      if (sc[s].rel < sc[s].eps) {
+        sc[s].status = MTCOCR_OK;
+        sc[s].iter = k - 1;
+        sc[s].last = k - 1;
+        stopped = true;
+      } else if (sc[s].stall_window > 0) {
+        // inner solves of the refinement stop when the smoothed residual has not halved within
+        // stall_window iterations (FP32 has reached its attainable accuracy)
+        if (sc[s].rel < 0.5 * sc[s].best_rel) {
+          sc[s].best_rel = sc[s].rel;
+          sc[s].last_improve = k - 1;
+        } else if (k - 1 - sc[s].last_improve > sc[s].stall_window) {
+          sc[s].status = MTCOCR_MAXITER;
+          sc[s].last = k - 1;
+          stopped = true;
+        }
+      }
     }
-    sc[s].w = w;
-    sc[s].mode = mode;
-  }
-}
-
-// after k_mrs2: residual and stop test (rci.cpp:66-77)
-template <int SP>
-__global__ __launch_bounds__(kScalarThreads) void s_stop(const double* part, int nblk,
-                                                         SysScalars* sc, int* nrun) {
-  __shared__ double tot[SP];
-  reduce_partials<SP>(part, nblk, tot);
-  const int s = threadIdx.x;
-  if (s < SP && sc[s].status == ST_RUNNING) {
-    const double ss = tot[s];
-    if (sc[s].mode == MRS_BLEND) sc[s].res = sqrt(ss);
-    else if (sc[s].mode == MRS_COPY) sc[s].res = ss > 0.0 ? sqrt(ss) : 0.0;
-    sc[s].rel = sc[s].res / sc[s].r_old;
-    if (sc[s].rel < sc[s].eps) {
-      sc[s].status = MTCOCR_OK;
-    } else if (sc[s].stall_window > 0) {
-      // inner solves of the refinement stop when the smoothed residual has not halved within
-      // stall_window iterations (FP32 has reached its attainable accuracy)
-      if (sc[s].rel < 0.5 * sc[s].best_rel) {
-        sc[s].best_rel = sc[s].rel;
-        sc[s].last_improve = sc[s].iter;
-      } else if (sc[s].iter - sc[s].last_improve > sc[s].stall_window) {
-        sc[s].status = MTCOCR_MAXITER;
+    if (!stopped) {
+      if (sc[s].halt) {
+        sc[s].status = sc[s].halt;  // x of iteration k - 1 is the answer
+        sc[s].last = k - 1;
+      } else {
+        double w = (fabs(uu) < 1e-100) ? 0.0 : -su / uu;
+        int mode;
+        if (w < 0) {
+          w = 0;
+          mode = MRS_SKIP;
+        } else if (w > 1) {
+          w = 1;
+          mode = MRS_COPY;
+        } else {
+          mode = MRS_BLEND;
+        }
+        sc[s].w = w;
+        sc[s].mode = mode;
       }
     }
   }
@@ -886,7 +900,7 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
       sc[s].zs_im = zi;
       // the reference tests aw here, not zs (pcocr_rci.cpp:328)
       if (fabs(sc[s].aw_re) < 1e-40 && fabs(sc[s].aw_im) < 1e-40) {
-        sc[s].status = MTCOCR_DIV_BY_ZERO;
+        if (!sc[s].halt) sc[s].halt = MTCOCR_DIV_BY_ZERO;
       } else {
         double qr, qi;
         cdiv(zr, zi, sc[s].zs1_re, sc[s].zs1_im, qr, qi);
@@ -894,9 +908,11 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
         sc[s].be_im = qi;
       }
     }
-    if (sc[s].status == ST_RUNNING) {
+    if (sc[s].status == ST_RUNNING && !sc[s].halt) {
       sc[s].iter += 1;
-      if (sc[s].iter > maxiter) sc[s].status = MTCOCR_MAXITER;
+      // the stop test of the iteration just finished is still pending (s_mrs_stop of the next
+      // iteration decides it first)
+      if (sc[s].iter > maxiter) sc[s].halt = MTCOCR_MAXITER;
     }
   }
   __syncthreads();
@@ -1116,7 +1132,7 @@ struct mtcocr_handle {
   DevBuf omega, scal, nrun, nrun2, part, part2;
   DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
   // COCR work vectors (type depends on the precision of the last solve)
-  DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vb;
+  DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
   int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
@@ -1150,7 +1166,7 @@ struct mtcocr_handle {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
                            &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
-                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn,
+                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2,
                            &vx, &vb};
     int64_t t = 0;
     for (auto* b : all) t += (int64_t)b->bytes;
@@ -1275,7 +1291,7 @@ struct Engine {
   mtcocr_handle* h;
   EllView<double> A;
   GView<T> G;
-  T2 *x, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *dn;
+  T2 *x, *x2, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *dn;
   DiagA da;
   SysScalars* sc;
   int* nrun;
@@ -1289,6 +1305,7 @@ struct Engine {
     nb = h->nb;
     m = h->m;
     x = h->vx.as<T2>();
+    x2 = h->vx2.as<T2>();
     b = h->vb.as<T2>();
     r = h->vr.as<T2>();
     p = h->vp.as<T2>();
@@ -1359,19 +1376,18 @@ struct Engine {
     s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
 
-  // one COCR iteration (stages 5-8)
-  void iteration(hipStream_t st, int maxiter) {
+  // one COCR iteration (stages 5-8); `odd` = the iteration's index is odd (it reads x from
+  // buffer 0 and writes buffer 1), which alternates within a captured chunk of iterations
+  void iteration(hipStream_t st, int maxiter, bool odd) {
     k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
     k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, s, p, w, da, help, sc, nrun, part);
     int nr;
     const double* pp = partials<SP * 2>(st, grid_e, nr);
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    k_xr_mrs1<T, SP><<<grid_e, kBlock, 0, st>>>(x, p, r, a, sm, sc, nrun, part, nb);
-    pp = partials<SP * 2>(st, grid_e, nr);
-    s_mrs<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc);
-    k_mrs2<T, SP><<<grid_e, kBlock, 0, st>>>(x, r, ym, sm, s, w, sc, nrun, part, nb);
-    pp = partials<SP>(st, grid_e, nr);
-    s_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
+    k_xr_fused<T, SP><<<grid_e, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w, ym,
+                                                 sm, sc, nrun, part, nb);
+    pp = partials<SP * 3>(st, grid_e, nr);
+    s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     pp = partials<SP * 2>(st, grid_s, nr);
     s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
@@ -1391,7 +1407,7 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
   const size_t nm = (size_t)std::max(h->m, 1) * sp * es;
   int rc = 0;
   DevBuf* vecs[] = {&h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw, &h->vym, &h->vsm, &h->vx,
-                    &h->vb};
+                    &h->vx2, &h->vb};
   for (DevBuf* b : vecs)
     if ((rc = ensure(h, *b, nv))) return rc;
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
@@ -1421,7 +1437,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter) {
       }
       hipGraph_t g;
       HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter);
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, (c & 1) == 0);
       HIPC(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -1448,7 +1464,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter) {
     if (use_graphs()) {
       HIPC(hipGraphLaunch(h->graph, st));
     } else {
-      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter);
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, (c & 1) == 0);
     }
     HIPC(hipGetLastError());
     launched += chunk;
@@ -1494,6 +1510,12 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   HIPC(hipGetLastError());
   rc = run_iterations<double, SP>(h, E, maxiter);
   if (rc) return rc;
+  {
+    const long long nv = (long long)h->nb * SP;
+    const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
+    k_final_x<double, SP><<<gv, 256, 0, st>>>(E.x, E.x2, E.sc, h->nb);
+    HIPC(hipGetLastError());
+  }
   HIPC(hipEventRecord(h->ev1, st));
   HIPC(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -1652,8 +1674,8 @@ int set_batch(mtcocr_handle* h, int nsys, const double* omega) {
   if ((rc = ensure(h, h->scal, sizeof(SysScalars) * 64))) return rc;
   if ((rc = ensure(h, h->nrun, 64))) return rc;
   if ((rc = ensure(h, h->nrun2, 64))) return rc;
-  if ((rc = ensure(h, h->part, sizeof(double) * (size_t)kMaxBlocks * 64 * 2))) return rc;
-  if ((rc = ensure(h, h->part2, sizeof(double) * (size_t)(kMaxBlocks / kFold) * 64 * 2)))
+  if ((rc = ensure(h, h->part, sizeof(double) * (size_t)kMaxBlocks * 64 * 3))) return rc;
+  if ((rc = ensure(h, h->part2, sizeof(double) * (size_t)(kMaxBlocks / kFold) * 64 * 3)))
     return rc;
   const size_t v64 = (size_t)h->nb * sp * sizeof(double2);
   if ((rc = ensure(h, h->b64, v64))) return rc;
@@ -2067,7 +2089,7 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 //   SpMV            layout (sliced ELL k_spmv, or row pack k_spmv_rg for 4..64 systems) x cap
 //                   (the cap also sets the partial count of its reductions)
 //   node pass       k_pc_node cap
-//   edge streams    k_pc_edge_iter cap, which stands for k_xr_mrs1 and k_mrs2 as well
+//   edge streams    k_pc_edge_iter cap, which stands for k_xr_fused as well
 // Only work buffers are written; the resident right-hand sides and solutions are untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms) {
