@@ -751,16 +751,40 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
   block_partials<SP, 3>(v, partials);
 }
 
-// x of each system's last iteration into x0: iteration k writes buffer k % 2
-template <class T, int SP>
-__global__ void k_final_x(typename Vec2<T>::type* __restrict__ x0,
-                          const typename Vec2<T>::type* __restrict__ x1,
-                          const SysScalars* __restrict__ sc, int nb) {
+
+// ---- batch compaction: narrowing the batch to the systems still running -------------------
+
+// dst[row][j] = src[row][map[j]] for the sp_to slots of a narrower batch (map[j] < 0: zero)
+template <class T2>
+__global__ void k_repack(const T2* __restrict__ src, int sp_from, T2* __restrict__ dst,
+                         int sp_to, const int* __restrict__ map, long long rows) {
+  const long long N = rows * sp_to;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long row = t / sp_to;
+    const int m = map[t % sp_to];
+    T2 v;
+    v.x = 0;
+    v.y = 0;
+    if (m >= 0) v = src[row * sp_from + m];
+    dst[t] = v;
+  }
+}
+
+// x64[row][sel[j]] = x of slot j's last iteration (buffer last % 2), for the slots with
+// sel[j] >= 0 (the systems that finished, in the caller's batch order)
+template <int SP>
+__global__ void k_scatter_x(double2* __restrict__ x64, int sp0, const double2* __restrict__ x0,
+                            const double2* __restrict__ x1, const SysScalars* __restrict__ sc,
+                            const int* __restrict__ sel, int nb) {
   const long long N = (long long)nb * SP;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N;
        t += (long long)gridDim.x * blockDim.x) {
-    const int s = int(t % SP);
-    if (sc[s].status != ST_INACTIVE && (sc[s].last & 1)) x0[t] = x1[t];
+    const int j = int(t % SP);
+    const int o = sel[j];
+    if (o < 0) continue;
+    const long long row = t / SP;
+    x64[row * sp0 + o] = (sc[j].last & 1) ? x1[t] : x0[t];
   }
 }
 
@@ -1133,6 +1157,9 @@ struct mtcocr_handle {
   DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
+  // the same vectors for a narrowed batch (swapped with the above by compaction)
+  DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
+  DevBuf cmap;  // slot maps of compaction / scatter
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
   int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
@@ -1166,7 +1193,8 @@ struct mtcocr_handle {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
                            &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
-                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2,
+                           &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2, &cvr, &cvp, &cvs, &cvz, &cva, &cvw, &cvym,
+                           &cvsm, &cvx, &cvx2, &cmap,
                            &vx, &vb};
     int64_t t = 0;
     for (auto* b : all) t += (int64_t)b->bytes;
@@ -1424,7 +1452,7 @@ bool use_graphs() {
 
 // Run COCR iterations until every system of the engine has left the RUNNING state.
 template <class T, int SP>
-int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter) {
+int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_below = 0) {
   hipStream_t st = h->stream;
   const int prec = std::is_same<T, double>::value ? 0 : 1;
   const int chunk = 8;
@@ -1456,6 +1484,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter) {
     HIPC(hipMemcpyAsync(h->h_flags, E.nrun, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (h->h_flags[0] == 0) break;
+    if (h->h_flags[0] <= stop_below) return 1;  // the caller narrows the batch
     if (launched > cap) {
       std::snprintf(g_err, sizeof(g_err), "solver did not retire its systems after %lld iterations",
                     launched);
@@ -1484,6 +1513,104 @@ void set_eps_status(mtcocr_handle* h, std::vector<SysScalars>& hs, int nsys, int
   (void)h;
 }
 
+// Outcome of each system of an FP64 batch (caller's batch order), recorded when it leaves.
+struct Fp64Run {
+  int sp0 = 0;
+  std::vector<SysScalars> final_sc;
+  std::vector<char> done;
+};
+
+template <class F>
+int with_sp(int sp, F&& f);
+
+// Runs the FP64 iterations of a batch of SP slots (slot j holds the caller's system orig[j]).
+// When the running systems fit into half the width, the finished ones are written to x64,
+// the running ones are repacked into a batch of the next power of two and the iterations
+// continue there: systems that converge late no longer pay for the width of the whole batch.
+// Iteration counters, MRS state and x parity move with the systems; D_N^-1 is recomputed for
+// the new order. Results are those of the wide batch up to the regrouped dot products.
+template <int SP>
+int fp64_phase(mtcocr_handle* h, int maxiter, const std::vector<int>& orig,
+               const std::vector<double>& om, Fp64Run& R) {
+  hipStream_t st = h->stream;
+  int rc;
+  Engine<double, SP> E(h);
+  const int narrowed = run_iterations<double, SP>(h, E, maxiter, SP > 1 ? SP / 2 : 0);
+  if (narrowed < 0) return narrowed;
+  std::vector<SysScalars> hs(SP);
+  HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  std::vector<int> sel(SP, -1), running;
+  for (int j = 0; j < SP; ++j) {
+    if (orig[j] < 0 || hs[j].status == ST_INACTIVE) continue;
+    if (hs[j].status == ST_RUNNING) {
+      running.push_back(j);
+      continue;
+    }
+    sel[j] = orig[j];
+    R.final_sc[orig[j]] = hs[j];
+    R.done[orig[j]] = 1;
+  }
+  if ((rc = ensure(h, h->cmap, sizeof(int) * 128))) return rc;
+  const long long nv = (long long)h->nb * SP;
+  const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
+  HIPC(hipMemcpyAsync(h->cmap.p, sel.data(), sizeof(int) * SP, hipMemcpyHostToDevice, st));
+  k_scatter_x<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), R.sp0, E.x, E.x2, E.sc,
+                                      h->cmap.as<int>(), h->nb);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(st));
+  if (narrowed == 0 || running.empty()) return 0;
+  // narrow the batch
+  const int sp2 = pad_sp((int)running.size());
+  std::vector<int> map(sp2, -1), orig2(sp2, -1);
+  std::vector<double> om2(sp2, 0.0);
+  std::vector<SysScalars> sc2(sp2);
+  for (int k = 0; k < sp2; ++k) {
+    std::memset(&sc2[k], 0, sizeof(SysScalars));
+    sc2[k].status = sc2[k].outer_status = ST_INACTIVE;
+  }
+  for (int k = 0; k < (int)running.size(); ++k) {
+    map[k] = running[k];
+    orig2[k] = orig[running[k]];
+    om2[k] = om[running[k]];
+    sc2[k] = hs[running[k]];
+  }
+  DevBuf* mains[] = {&h->vx, &h->vx2, &h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw,
+                     &h->vym, &h->vsm};
+  DevBuf* spares[] = {&h->cvx, &h->cvx2, &h->cvr, &h->cvp, &h->cvs, &h->cvz, &h->cva, &h->cvw,
+                      &h->cvym, &h->cvsm};
+  HIPC(hipMemcpyAsync(h->cmap.p, map.data(), sizeof(int) * sp2, hipMemcpyHostToDevice, st));
+  const long long nv2 = (long long)h->nb * sp2;
+  const int g2 = (int)std::min<long long>(4096, (nv2 + 255) / 256);
+  for (int i = 0; i < 10; ++i) {
+    if ((rc = ensure(h, *spares[i], sizeof(double2) * (size_t)nv2))) return rc;
+    k_repack<double2><<<g2, 256, 0, st>>>(mains[i]->as<double2>(), SP, spares[i]->as<double2>(),
+                                          sp2, h->cmap.as<int>(), h->nb);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(st));
+  for (int i = 0; i < 10; ++i) {
+    std::swap(mains[i]->p, spares[i]->p);
+    std::swap(mains[i]->bytes, spares[i]->bytes);
+  }
+  drop_graph(h);  // captured iterations hold the old buffers and width
+  const int nrun = (int)running.size();
+  HIPC(hipMemcpyAsync(h->scal.p, sc2.data(), sizeof(SysScalars) * sp2, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(h->omega.p, om2.data(), sizeof(double) * sp2, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(h->nrun.p, &nrun, sizeof(int), hipMemcpyHostToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  return with_sp(sp2, [&](auto spc) -> int {
+    constexpr int SP2 = decltype(spc)::value;
+    if constexpr (SP2 < SP) {
+      Engine<double, SP2> E2(h);
+      E2.setup_diagonals(st);
+      HIPC(hipGetLastError());
+      return fp64_phase<SP2>(h, maxiter, orig2, om2, R);
+    }
+    return MTCOCR_ERR_STATE;
+  });
+}
+
 // Plain FP64 COCR on the systems of `mask` (all when null), warm-started from x64.
 template <int SP>
 int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
@@ -1497,6 +1624,8 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   if (mask)
     for (int s = 0; s < h->nsys; ++s)
       if (!(*mask)[s]) hs[s].status = hs[s].outer_status = ST_INACTIVE;
+  std::vector<double> om0(SP);
+  HIPC(hipMemcpyAsync(om0.data(), h->omega.p, sizeof(double) * SP, hipMemcpyDeviceToHost, st));
   HIPC(hipMemcpyAsync(E.sc, hs.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
   int one = 1;
   HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
@@ -1508,27 +1637,40 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   E.setup_diagonals(st);
   E.prologue(st, maxiter);
   HIPC(hipGetLastError());
-  rc = run_iterations<double, SP>(h, E, maxiter);
+  Fp64Run R;
+  R.sp0 = SP;
+  R.final_sc.assign(SP, SysScalars());
+  R.done.assign(SP, 0);
+  std::vector<int> orig(SP, -1);
+  for (int s = 0; s < h->nsys; ++s) orig[s] = s;
+  rc = fp64_phase<SP>(h, maxiter, orig, om0, R);
   if (rc) return rc;
   {
-    const long long nv = (long long)h->nb * SP;
-    const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
-    k_final_x<double, SP><<<gv, 256, 0, st>>>(E.x, E.x2, E.sc, h->nb);
-    HIPC(hipGetLastError());
+    // keep the wider set of work vectors in place for the next solve (their contents are
+    // spent: the answers are in x64)
+    DevBuf* mains[] = {&h->vx, &h->vx2, &h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw,
+                       &h->vym, &h->vsm};
+    DevBuf* spares[] = {&h->cvx, &h->cvx2, &h->cvr, &h->cvp, &h->cvs, &h->cvz, &h->cva,
+                        &h->cvw, &h->cvym, &h->cvsm};
+    for (int i = 0; i < 10; ++i)
+      if (spares[i]->bytes > mains[i]->bytes) {
+        std::swap(mains[i]->p, spares[i]->p);
+        std::swap(mains[i]->bytes, spares[i]->bytes);
+        drop_graph(h);
+      }
   }
   HIPC(hipEventRecord(h->ev1, st));
   HIPC(hipEventSynchronize(h->ev1));
   float ms = 0.f;
   HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  // solution back into x64, true residual in FP64
-  HIPC(hipMemcpyAsync(h->x64.p, E.x, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
-  HIPC(hipMemcpyAsync(hs.data(), E.sc, sizeof(SysScalars) * SP, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
+  // the batch in the caller's order again, true residual of x64 in FP64
+  HIPC(hipMemcpyAsync(h->omega.p, om0.data(), sizeof(double) * SP, hipMemcpyHostToDevice, st));
   std::vector<SysScalars> tmp(hs);
   for (int s = 0; s < SP; ++s) tmp[s].status = (s < h->nsys) ? ST_RUNNING : ST_INACTIVE;
   HIPC(hipMemcpyAsync(E.sc, tmp.data(), sizeof(SysScalars) * SP, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(E.nrun, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  E.template spmv<1>(st, E.x, h->r64.as<double2>(), E.b, (double2*)nullptr, (double2*)nullptr);
+  E.template spmv<1>(st, h->x64.as<double2>(), h->r64.as<double2>(), h->b64.as<double2>(),
+                     (double2*)nullptr, (double2*)nullptr);
   HIPC(hipGetLastError());
   std::vector<double> partials((size_t)E.grid_s * SP * 2);
   HIPC(hipMemcpyAsync(partials.data(), E.part, partials.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1540,23 +1682,19 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
       bb += partials[((size_t)b * SP + s) * 2];
       rr += partials[((size_t)b * SP + s) * 2 + 1];
     }
+    const SysScalars& f = R.final_sc[s];
     mtcocr_stats& o = stats[s];
-    o.req = hs[s].status == ST_RUNNING ? MTCOCR_MAXITER : hs[s].status;
-    o.iter = hs[s].iter;
+    o.req = !R.done[s] ? MTCOCR_ERR_STATE : f.status;
+    o.iter = f.iter;
     o.outer = 1;
     o.flags = 0;
-    o.residual_rel = hs[s].rel;
+    o.residual_rel = f.rel;
     o.true_rel_res = bb > 0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
     o.seconds = ms * 1e-3;
   }
   return 0;
 }
 
-// Mixed precision: FP32 inner COCR (FP32 vectors and preconditioner, FP64 matrix) inside an
-// FP64 iterative refinement on the true residual. A system whose refinement pass gains less
-// than 10x (the inner FP32 solve has reached its attainable accuracy: at low frequencies the
-// galvanic part of the field lies below FP32 resolution) is finished by the FP64 solver,
-// warm-started from the refined iterate.
 template <int SP>
 int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) {
   hipStream_t st = h->stream;
