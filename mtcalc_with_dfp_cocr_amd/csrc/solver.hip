@@ -644,11 +644,11 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
   }
 }
 
-// Iteration form (stages 5-6): a = z + beta*a, p = s + beta*p, w = M^-1 a, partial (a, w).
+// Iteration form (stages 5-6): a = z + beta*a, w = M^-1 a, partial (a, w). (p = s + beta*p of
+// the same stage is formed by k_xr_fused, which reads s and p anyway.)
 template <class T, int SP>
 __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     GView<T> G, const typename Vec2<T>::type* __restrict__ z, typename Vec2<T>::type* __restrict__ a,
-    const typename Vec2<T>::type* __restrict__ s_vec, typename Vec2<T>::type* __restrict__ p,
     typename Vec2<T>::type* __restrict__ w, DiagA da,
     const typename Vec2<T>::type* __restrict__ help, const SysScalars* __restrict__ sc,
     const int* __restrict__ nrun, double* partials) {
@@ -666,7 +666,6 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     const int e = int(t / SP);
     const double br = sc[s].be_re, bi = sc[s].be_im;
     const T2 an = caxpy(z[t], a[t], br, bi);
-    const T2 pn = caxpy(s_vec[t], p[t], br, bi);
     const T2 d = diag_inv<T>(da, e, s);
     T yr = an.x * d.x - an.y * d.y;
     T yi = an.x * d.y + an.y * d.x;
@@ -681,7 +680,6 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     o.x = yr;
     o.y = yi;
     a[t] = an;
-    p[t] = pn;
     w[t] = o;
     v[0] += double(an.x) * double(yr) - double(an.y) * double(yi);
     v[1] += double(an.x) * double(yi) + double(an.y) * double(yr);
@@ -693,15 +691,16 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
 //   1. the MRS step of iteration k - 1, whose weight s_mrs_stop computed from the previous
 //      pass: ym += w (x - ym), sm += w (r - sm) (or a copy, or nothing), partial ||sm||^2 for
 //      the deferred stop test of k - 1;
-//   2. iteration k: x += alpha p (into the other x buffer, so that x of k - 1 survives if k - 1
-//      turns out to be the last iteration), r -= alpha a, s -= alpha w, and the partials
-//      (sm, u), (u, u) of u = r - sm for the MRS weight of k.
+//   2. iteration k: p = s + beta p (stage 5), x += alpha p (into the other x buffer, so that x of
+//      k - 1 survives if k - 1 turns out to be the last iteration), r -= alpha a, s -= alpha w,
+//      and the partials (sm, u), (u, u) of u = r - sm for the MRS weight of k.
 // Step 2 is skipped by a system that halts in iteration k. Deferring the MRS step saves
-// re-reading x and r: 8 vector reads and 5 writes instead of 11 and 5 in two passes.
+// re-reading x and r, and forming p here saves the preconditioner pass reading s and p: 8 vector
+// reads and 6 writes instead of 13 and 6.
 template <class T, int SP>
 __global__ __launch_bounds__(kBlock) void k_xr_fused(
     const typename Vec2<T>::type* __restrict__ xin, typename Vec2<T>::type* __restrict__ xout,
-    const typename Vec2<T>::type* __restrict__ p, typename Vec2<T>::type* __restrict__ r,
+    typename Vec2<T>::type* __restrict__ p, typename Vec2<T>::type* __restrict__ r,
     const typename Vec2<T>::type* __restrict__ a, typename Vec2<T>::type* __restrict__ s_vec,
     const typename Vec2<T>::type* __restrict__ w, typename Vec2<T>::type* __restrict__ ym,
     typename Vec2<T>::type* __restrict__ sm, const SysScalars* __restrict__ sc,
@@ -739,9 +738,12 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
     }
     if (sc[s].halt) continue;
     const double ar = sc[s].al_re, ai = sc[s].al_im;
-    const T2 xn = caxpy(xv, p[t], ar, ai);
+    const T2 sv = s_vec[t];
+    const T2 pn = caxpy(sv, p[t], sc[s].be_re, sc[s].be_im);
+    const T2 xn = caxpy(xv, pn, ar, ai);
     const T2 rn = caxpy(rv, a[t], -ar, -ai);
-    s_vec[t] = caxpy(s_vec[t], w[t], -ar, -ai);
+    s_vec[t] = caxpy(sv, w[t], -ar, -ai);
+    p[t] = pn;
     xout[t] = xn;
     r[t] = rn;
     const T ur = rn.x - smv.x, ui = rn.y - smv.y;
@@ -1408,7 +1410,7 @@ struct Engine {
   // buffer 0 and writes buffer 1), which alternates within a captured chunk of iterations
   void iteration(hipStream_t st, int maxiter, bool odd) {
     k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
-    k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, s, p, w, da, help, sc, nrun, part);
+    k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, w, da, help, sc, nrun, part);
     int nr;
     const double* pp = partials<SP * 2>(st, grid_e, nr);
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
@@ -2313,7 +2315,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
                                                                     E.sc, E.nrun);
               } else {
-                k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.s, E.p, E.w,
+                k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.w,
                                                                    E.da, E.help, E.sc, E.nrun,
                                                                    E.part);
                 int nr;
