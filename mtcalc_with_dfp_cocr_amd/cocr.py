@@ -141,8 +141,8 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    LAYOUTS = {"ell": 0, "rowpack": 1}
-    LAYOUT_NAMES = ("ell", "rowpack")
+    LAYOUTS = {"ell": 0, "rowpack": 1, "rowblock": 2}
+    LAYOUT_NAMES = ("ell", "rowpack", "rowblock")
 
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
                    spmv_layout: str | None = None) -> None:

@@ -171,6 +171,48 @@ int build_rows(int nb, const int* ig, const int* jg, GetLower lower, GetDiag dia
     for (; kr < out.nr[s]; ++kr) out.colr[out.offr[s] + 64LL * kr + lane] = i;
   }
   // lanes past nb in the last slice keep column 0 and zero values
+  // row blocks of kBlockRows rows and their halos
+  {
+    constexpr int R = SlicedEll::kBlockRows;
+    const int nblk = (nb + R - 1) / R;
+    out.hoff.assign(nblk + 1, 0);
+    out.halo.clear();
+    out.halo.reserve((size_t)nb * 10);
+    out.lcol.assign(out.pcol.size(), 0);
+    out.ldiag.assign(nb, 0);
+    out.halo_max = 0;
+    std::vector<int> stamp(nb, -1), pos(nb, 0), cols;
+    for (int k = 0; k < nblk && out.halo_max <= SlicedEll::kHaloCap; ++k) {
+      const int r0 = k * R, r1 = std::min(nb, r0 + R);
+      cols.clear();
+      auto take = [&](int c) {
+        if (stamp[c] != k) {
+          stamp[c] = k;
+          cols.push_back(c);
+        }
+      };
+      for (int q = r0; q < r1; ++q) {
+        take(q);
+        for (int e = out.rp[q]; e < out.rp[q + 1]; ++e) take(out.pcol[e]);
+      }
+      std::sort(cols.begin(), cols.end());
+      for (size_t i = 0; i < cols.size(); ++i) pos[cols[i]] = (int)i;
+      for (int q = r0; q < r1; ++q) {
+        out.ldiag[q] = (unsigned short)pos[q];
+        for (int e = out.rp[q]; e < out.rp[q + 1]; ++e)
+          out.lcol[e] = (unsigned short)pos[out.pcol[e]];
+      }
+      out.halo.insert(out.halo.end(), cols.begin(), cols.end());
+      out.hoff[k + 1] = (int)out.halo.size();
+      out.halo_max = std::max(out.halo_max, (int)cols.size());
+    }
+    if (out.halo_max > SlicedEll::kHaloCap) {
+      out.hoff.clear();
+      out.halo.clear();
+      out.lcol.clear();
+      out.ldiag.clear();
+    }
+  }
   return 0;
 }
 

@@ -30,6 +30,16 @@ struct SlicedEll {
   // columns), whose b values are [rb[i], rb[i+1]) of pb, then its real ones (ascending).
   std::vector<int> rp, rb, pcol;
   std::vector<double> pa, pb;
+  // Row blocks for the halo-staged SpMV: block k is rows [16k, 16k + 16); its distinct columns
+  // (the rows themselves included) are halo[hoff[k] .. hoff[k+1]), ascending. Every row-pack
+  // entry also has its column's position in its block's halo (lcol, parallel to pcol), every
+  // row its own position (ldiag). halo_max is the largest halo; blocks are not built (hoff
+  // empty) when one would exceed kHaloCap columns.
+  static constexpr int kBlockRows = 16;
+  static constexpr int kHaloCap = 512;
+  std::vector<int> hoff, halo;
+  std::vector<unsigned short> lcol, ldiag;
+  int halo_max = 0;
 };
 
 struct GradientEll {

@@ -80,6 +80,19 @@ struct RowView {
   int nb;
 };
 
+// The row pack seen through 16-row blocks: block k's distinct columns are
+// halo[hoff[k] .. hoff[k+1]); lcol (parallel to the row pack's columns) and ldiag give each
+// entry's and each row's position in its block's halo.
+struct BlkView {
+  const int *rp, *rb, *hoff, *halo;
+  const unsigned short *lcol, *ldiag;
+  const double *a, *b, *da, *db;
+  int nb, nblk;
+};
+
+constexpr int kRowBlock = 16;       // rows per block of the halo-staged SpMV
+constexpr int kHaloStage = 192;     // largest block halo it stages (columns)
+
 template <class T>
 struct GView {
   const int *gcol, *gtcol;
@@ -314,7 +327,7 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
 // r = b - A x with the partials ||b||^2, ||r||^2 and the MRS initialisation (MODE 1, 2).
 template <class T, int MODE>
 __device__ inline void spmv_store(long long t, double yr, double yi,
-                                  const typename Vec2<T>::type* __restrict__ x,
+                                  const typename Vec2<T>::type* __restrict__ xsrc, long long tx,
                                   typename Vec2<T>::type* __restrict__ y,
                                   const typename Vec2<T>::type* __restrict__ b,
                                   typename Vec2<T>::type* __restrict__ ym,
@@ -325,10 +338,10 @@ __device__ inline void spmv_store(long long t, double yr, double yi,
     o.x = T(yr);
     o.y = T(yi);
     y[t] = o;
-    const T2 xs = x[t];
+    const T2 xt = xsrc[tx];
     // the dot uses the stored (rounded) product, as the reference's ScalCmplx(z, s) does
-    v[0] += double(o.x) * double(xs.x) - double(o.y) * double(xs.y);
-    v[1] += double(o.x) * double(xs.y) + double(o.y) * double(xs.x);
+    v[0] += double(o.x) * double(xt.x) - double(o.y) * double(xt.y);
+    v[1] += double(o.x) * double(xt.y) + double(o.y) * double(xt.x);
   } else {
     const T2 bv = b[t];
     T2 o;
@@ -338,7 +351,7 @@ __device__ inline void spmv_store(long long t, double yr, double yi,
     v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
     v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
     if (ym) {
-      ym[t] = x[t];
+      ym[t] = xsrc[tx];
       sm[t] = o;
     }
   }
@@ -401,7 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<double> A, const double
         yi += a * double(xv.y);
       }
     }
-    spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
+    spmv_store<T, MODE>(t, yr, yi, x, t, y, b, ym, sm, v);
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -565,9 +578,181 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* ome
         }
         rg_chunk<T, SP>(cur.end - k0, c, a, bb, w, run, x, s, yr, yi);
       }
-      if (run) spmv_store<T, MODE>(t, yr, yi, x, y, b, ym, sm, v);
+      if (run) spmv_store<T, MODE>(t, yr, yi, x, t, y, b, ym, sm, v);
     }
     cur = nxt;
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// rg_chunk with x read from the block's LDS copy (lc = the entry's halo position). A helper of
+// its own: sharing rg_chunk between a global and an LDS source changes how the compiler
+// schedules k_spmv_rg (100 -> 183 VGPRs at 16 systems).
+template <class T, int SP>
+__device__ inline void lds_chunk(int cnt, int lc, double a, double bb, double w,
+                                 const typename Vec2<T>::type* xs, int s, double& yr,
+                                 double& yi) {
+  using T2 = typename Vec2<T>::type;
+  auto step = [&](auto J) {
+    const int cj = grp_bcast<SP, J>(lc);
+    const double aj = grp_bcast_d<SP, J>(a);
+    const double bj = grp_bcast_d<SP, J>(bb) * w;
+    const T2 xv = xs[cj * SP + s];
+    const double xr = double(xv.x), xm = double(xv.y);
+    yr += aj * xr - bj * xm;
+    yi += aj * xm + bj * xr;
+  };
+  if (cnt >= SP) {
+    static_for<0, SP>(step);
+  } else {
+    static_for<0, SP>([&](auto J) {
+      if (J < cnt) step(J);
+    });
+  }
+}
+
+// Halo-staged SpMV over 16-row blocks (batch widths 4..16). A workgroup copies the x values
+// of a block's distinct columns (its halo, every system) into LDS with coalesced loads, then
+// each row group (SP lanes, one per system) walks its row as k_spmv_rg does, reading x from LDS
+// through the entries' 16-bit halo positions. x then crosses the vector memory path once per
+// block column instead of once per non-zero. 256/SP row groups cover 16/SP blocks at a time,
+// each with its own LDS region of hmax * SP values (hmax <= kHaloStage). The walk is software
+// pipelined: while a tile computes from LDS, the next tile's x values and row data and the
+// tile after's halo indices are in flight in registers. Same modes, outputs and partials as
+// k_spmv; the diagonal is applied first.
+template <class T, int SP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const double* omega,
+                                                     const typename Vec2<T>::type* __restrict__ x,
+                                                     typename Vec2<T>::type* __restrict__ y,
+                                                     const typename Vec2<T>::type* __restrict__ b,
+                                                     typename Vec2<T>::type* __restrict__ ym,
+                                                     typename Vec2<T>::type* __restrict__ sm,
+                                                     const SysScalars* __restrict__ sc,
+                                                     const int* __restrict__ nrun,
+                                                     double* partials) {
+  static_assert(SP >= 4 && SP <= 16 && kBlock % (SP * kRowBlock) == 0, "4..16 systems");
+  using T2 = typename Vec2<T>::type;
+  extern __shared__ __align__(16) unsigned char blk_lds[];
+  if (*nrun == 0) return;
+  constexpr int BW = kBlock / (SP * kRowBlock);  // blocks per pass
+  constexpr int BT = SP * kRowBlock;             // threads per block
+  constexpr int NI = kHaloStage / kRowBlock;     // staging slots per thread
+  const int s = int(threadIdx.x % SP);
+  const int sub = int(threadIdx.x / BT);
+  const int rl = int(threadIdx.x % BT) / SP;     // row within the block
+  T2* xs = reinterpret_cast<T2*>(blk_lds) + (size_t)sub * hmax * SP;
+  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
+  const double w = omega[s];
+  double v[2] = {0.0, 0.0};
+  const long long ntiles = ((long long)A.nblk + BW - 1) / BW;
+  TileWalk tw(ntiles);
+
+  // halo slot q of this thread is column rl + 16 q of its block; -1 past the halo
+  int ci[NI];
+  auto load_idx = [&](long long tile) {
+    const long long blk = tile * BW + sub;
+    int h0 = 0, hn = 0;
+    if (blk < A.nblk) {
+      h0 = A.hoff[blk];
+      hn = A.hoff[blk + 1] - h0;
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+      const int i = rl + q * kRowBlock;
+      ci[q] = i < hn ? A.halo[h0 + i] : -1;
+    }
+  };
+  T2 xv[NI];
+  auto load_x = [&]() {
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+      xv[q].x = T(0);
+      xv[q].y = T(0);
+      if (run && ci[q] >= 0) xv[q] = x[(size_t)ci[q] * SP + s];
+    }
+  };
+  unsigned int xmask = 0;  // slots of the staged tile that hold a halo column
+  auto mask_of = [&]() {
+    unsigned int m = 0;
+#pragma unroll
+    for (int q = 0; q < NI; ++q) m |= (ci[q] >= 0 ? 1u : 0u) << q;
+    return m;
+  };
+  struct Row {
+    int beg, end, bbeg, ncx, dl, c0, c1;
+    double a0, b0, a1, b1, dA, dB;
+  };
+  auto load_row = [&](long long tile, Row& R) {
+    const long long rowl = (tile * BW + sub) * kRowBlock + rl;
+    R.beg = R.end = R.bbeg = R.ncx = R.dl = R.c0 = R.c1 = 0;
+    R.a0 = R.b0 = R.a1 = R.b1 = R.dA = R.dB = 0.0;
+    if (rowl < A.nb) {
+      const int row = int(rowl);
+      R.beg = A.rp[row];
+      R.end = A.rp[row + 1];
+      R.bbeg = A.rb[row];
+      R.ncx = A.rb[row + 1] - R.bbeg;
+      R.dl = A.ldiag[row];
+      R.dA = A.da[row];
+      R.dB = A.db[row] * w;
+      if (R.beg + s < R.end) {
+        R.c0 = A.lcol[R.beg + s];
+        R.a0 = A.a[R.beg + s];
+        if (s < R.ncx) R.b0 = A.b[R.bbeg + s];
+      }
+      if (R.beg + SP + s < R.end) {
+        R.c1 = A.lcol[R.beg + SP + s];
+        R.a1 = A.a[R.beg + SP + s];
+        if (SP + s < R.ncx) R.b1 = A.b[R.bbeg + SP + s];
+      }
+    }
+  };
+
+  Row cur, nxt;
+  long long tile = tw.first;
+  if (tile < tw.last) {
+    load_idx(tile);
+    xmask = mask_of();
+    load_x();
+    load_row(tile, cur);
+    if (tile + tw.step < tw.last) load_idx(tile + tw.step);
+  }
+  for (; tile < tw.last; tile += tw.step) {
+#pragma unroll
+    for (int q = 0; q < NI; ++q)
+      if (xmask & (1u << q)) xs[(rl + q * kRowBlock) * SP + s] = xv[q];
+    __syncthreads();
+    const long long nt = tile + tw.step;
+    if (nt < tw.last) {  // ci holds tile nt's halo
+      xmask = mask_of();
+      load_x();
+      load_row(nt, nxt);
+      if (nt + tw.step < tw.last) load_idx(nt + tw.step);
+    }
+    const long long rowl = (tile * BW + sub) * kRowBlock + rl;
+    if (rowl < A.nb) {
+      const long long t = rowl * SP + s;
+      const T2 xt = xs[cur.dl * SP + s];
+      double yr = cur.dA * double(xt.x) - cur.dB * double(xt.y);
+      double yi = cur.dA * double(xt.y) + cur.dB * double(xt.x);
+      const int len = cur.end - cur.beg;
+      lds_chunk<T, SP>(len, cur.c0, cur.a0, cur.b0, w, xs, s, yr, yi);
+      if (len > SP) lds_chunk<T, SP>(len - SP, cur.c1, cur.a1, cur.b1, w, xs, s, yr, yi);
+      for (int k0 = cur.beg + 2 * SP; k0 < cur.end; k0 += SP) {  // rows longer than 2 chunks
+        const int l = k0 - cur.beg + s;
+        int c = 0;
+        double a = 0.0, bb = 0.0;
+        if (k0 + s < cur.end) {
+          c = A.lcol[k0 + s];
+          a = A.a[k0 + s];
+          if (l < cur.ncx) bb = A.b[cur.bbeg + l];
+        }
+        lds_chunk<T, SP>(cur.end - k0, c, a, bb, w, xs, s, yr, yi);
+      }
+      if (run) spmv_store<T, MODE>(t, yr, yi, xs, cur.dl * SP + s, y, b, ym, sm, v);
+    }
+    cur = nxt;
+    __syncthreads();  // the next tile's staging overwrites xs
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -1148,6 +1333,8 @@ struct mtcocr_handle {
   DevBuf offc, nc, offr, nr, colc, colr;
   DevBuf ac64, bc64, ar64, da64, db64;
   DevBuf rp, rb, pcol, pa, pb;  // row pack of the off-diagonal entries
+  DevBuf hoff, halo, lcol, ldiag;  // 16-row blocks of the row pack and their halos
+  int nblk = 0, halo_max = 0;      // halo_max 0: no row blocks
   // gradient
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0;
@@ -1169,7 +1356,8 @@ struct mtcocr_handle {
   // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
   int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
   // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg,
-  // batch widths 4..64)
+  // batch widths 4..64), 2 halo-staged row blocks (k_spmv_blk, batch widths 4..16; where it
+  // does not apply, the row pack runs)
   int spmv_layout[2][7];
   static int sp_index(int sp) {
     int k = 0;
@@ -1182,7 +1370,7 @@ struct mtcocr_handle {
         if (cs) cap_spmv[p][k] = cs;
         if (cv) cap_vec[p][k] = cv;
         if (cn) cap_node[p][k] = cn;
-        if (layout >= 0) spmv_layout[p][k] = (k >= 2) ? layout : 0;
+        if (layout >= 0) spmv_layout[p][k] = (k < 2) ? 0 : (k > 4 && layout > 1) ? 1 : layout;
       }
   }
   mtcocr_handle() {
@@ -1193,7 +1381,7 @@ struct mtcocr_handle {
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
-                           &db64, &rp, &rb, &pcol, &pa, &pb, &gcol, &gtcol, &gval64, &gtval64,
+                           &db64, &rp, &rb, &pcol, &pa, &pb, &hoff, &halo, &lcol, &ldiag, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2, &cvr, &cvp, &cvs, &cvz, &cva, &cvw, &cvym,
                            &cvsm, &cvx, &cvx2, &cmap,
@@ -1276,6 +1464,31 @@ RowView row_view(mtcocr_handle* h) {
   return v;
 }
 
+BlkView blk_view(mtcocr_handle* h) {
+  BlkView v;
+  v.rp = h->rp.as<int>();
+  v.rb = h->rb.as<int>();
+  v.hoff = h->hoff.as<int>();
+  v.halo = h->halo.as<int>();
+  v.lcol = h->lcol.as<unsigned short>();
+  v.ldiag = h->ldiag.as<unsigned short>();
+  v.a = h->pa.as<double>();
+  v.b = h->pb.as<double>();
+  v.da = h->da64.as<double>();
+  v.db = h->db64.as<double>();
+  v.nb = h->nb;
+  v.nblk = h->nblk;
+  return v;
+}
+
+// LDS bytes of one k_spmv_blk workgroup (16/SP blocks of halo_max x SP values), or 0 when
+// the matrix has no row blocks or one has more than kHaloStage columns
+template <class TV>
+int blk_lds_bytes(const mtcocr_handle* h) {
+  if (h->halo_max <= 0 || h->halo_max > kHaloStage) return 0;
+  return kRowBlock * h->halo_max * int(sizeof(typename Vec2<TV>::type));
+}
+
 // SpMV launch in the layout chosen for this batch width (vector type TV, which the outer FP64
 // residual of the mixed solver uses with an FP32 engine).
 template <class TV, int SP, int MODE>
@@ -1284,6 +1497,14 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  const typename Vec2<TV>::type* b, typename Vec2<TV>::type* ym,
                  typename Vec2<TV>::type* sm, const SysScalars* sc, const int* nrun,
                  double* part) {
+  if constexpr (SP >= 4 && SP <= 16) {
+    const int lds = blk_lds_bytes<TV>(h);
+    if (layout == 2 && lds > 0) {
+      k_spmv_blk<TV, SP, MODE><<<grid, kBlock, lds, st>>>(blk_view(h), h->halo_max, omega, x, y,
+                                                          b, ym, sm, sc, nrun, part);
+      return;
+    }
+  }
   if constexpr (SP >= 4) {
     if (layout >= 1) {
       k_spmv_rg<TV, SP, MODE><<<grid, kBlock, 0, st>>>(row_view(h), omega, x, y, b, ym, sm, sc,
@@ -1897,8 +2118,12 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
       (rc = upload(h, h->ar64, E.ar, st)) || (rc = upload(h, h->da64, E.diag_a, st)) ||
       (rc = upload(h, h->db64, E.diag_b, st)) || (rc = upload(h, h->rp, E.rp, st)) ||
       (rc = upload(h, h->rb, E.rb, st)) || (rc = upload(h, h->pcol, E.pcol, st)) ||
-      (rc = upload(h, h->pa, E.pa, st)) || (rc = upload(h, h->pb, E.pb, st)))
+      (rc = upload(h, h->pa, E.pa, st)) || (rc = upload(h, h->pb, E.pb, st)) ||
+      (rc = upload(h, h->hoff, E.hoff, st)) || (rc = upload(h, h->halo, E.halo, st)) ||
+      (rc = upload(h, h->lcol, E.lcol, st)) || (rc = upload(h, h->ldiag, E.ldiag, st)))
     return rc;
+  h->halo_max = E.hoff.empty() ? 0 : E.halo_max;
+  h->nblk = E.hoff.empty() ? 0 : int(E.hoff.size()) - 1;
   HIPC(hipStreamSynchronize(st));
   h->have_matrix = true;
   h->nsys = 0;
@@ -2214,11 +2439,25 @@ int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
   if (!h) return MTCOCR_ERR_ARG;
   auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
   if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
-  if (spmv_layout < -1 || spmv_layout > 1) return MTCOCR_ERR_ARG;
+  if (spmv_layout < -1 || spmv_layout > 2) return MTCOCR_ERR_ARG;
   h->set_caps(cap_spmv, cap_vec, cap_node, spmv_layout);
   drop_graph(h);  // the captured iterations hold the old grids
   return 0;
 }
+
+}  // extern "C"
+
+namespace {
+// the SpMV layout that runs for precision q at batch-width index k (row blocks fall back to
+// the row pack where they do not apply)
+int effective_layout(const mtcocr_handle* h, int q, int k) {
+  int l = h->spmv_layout[q][k];
+  if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
+  return l;
+}
+}  // namespace
+
+extern "C" {
 
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
                       int* cap_node, int* spmv_layout) {
@@ -2228,7 +2467,7 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
   if (cap_spmv) *cap_spmv = h->cap_spmv[q][k];
   if (cap_vec) *cap_vec = h->cap_vec[q][k];
   if (cap_node) *cap_node = h->cap_node[q][k];
-  if (spmv_layout) *spmv_layout = h->spmv_layout[q][k];
+  if (spmv_layout) *spmv_layout = effective_layout(h, q, k);
   return 0;
 }
 
@@ -2236,7 +2475,8 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 // frequencies) it times the kernel families whose launch is tunable, each at every candidate
 // workgroup cap, on zero-filled work vectors (the kernels' memory traffic does not depend on
 // the values), and keeps the fastest choice of each family:
-//   SpMV            layout (sliced ELL k_spmv, or row pack k_spmv_rg for 4..64 systems) x cap
+//   SpMV            layout (sliced ELL k_spmv, row pack k_spmv_rg for 4..64 systems, row
+//                   blocks k_spmv_blk for 4..16 systems) x cap
 //                   (the cap also sets the partial count of its reductions)
 //   node pass       k_pc_node cap
 //   edge streams    k_pc_edge_iter cap, which stands for k_xr_fused as well
@@ -2278,13 +2518,19 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         HIPC(hipGetLastError());
       }
       for (int fam = 0; fam < 3; ++fam) {
-        const int nlayouts = (fam == 0 && SP >= 4) ? 2 : 1;
+        const int nlayouts =
+            (fam != 0 || SP < 4) ? 1 : (SP <= 16 && blk_lds_bytes<T>(h) > 0) ? 3 : 2;
         for (int layout = 0; layout < nlayouts; ++layout) {
           // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
           // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
           // twice that
           int per_cu = 0;
-          if (fam == 0 && layout == 1) {
+          if (fam == 0 && layout == 2) {
+            if constexpr (SP >= 4 && SP <= 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_blk<T, SP, 0>), kBlock,
+                  blk_lds_bytes<T>(h)));
+          } else if (fam == 0 && layout == 1) {
             if constexpr (SP >= 4)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_rg<T, SP, 0>), kBlock, 0));
@@ -2363,7 +2609,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   if (cap_spmv) *cap_spmv = h->cap_spmv[q][ki];
   if (cap_node) *cap_node = h->cap_node[q][ki];
   if (cap_vec) *cap_vec = h->cap_vec[q][ki];
-  if (spmv_layout) *spmv_layout = h->spmv_layout[q][ki];
+  if (spmv_layout) *spmv_layout = effective_layout(h, q, ki);
   if (spmv_ms) *spmv_ms = best_ms[0];
   return 0;
 }
