@@ -88,6 +88,7 @@ struct BlkView {
   const unsigned short *lcol, *ldiag;
   const double *a, *b, *da, *db;
   int nb, nblk;
+  long long nhalo, nnz, nnzb;  // lengths of halo, a (and lcol), b
 };
 
 constexpr int kRowBlock = 16;       // rows per block of the halo-staged SpMV
@@ -419,36 +420,40 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<double> A, const double
   block_partials<SP, 2>(v, partials);
 }
 
-// Broadcast of lane J of each SP-lane row group to the whole group: DPP quad_perm (SP 4),
-// DPP row_newbcast within 16-lane rows (SP 8, 16), v_readlane (SP 32, 64). Every lane of a
-// group is active at the call (groups are whole rows).
-template <int SP, int J>
-__device__ inline int grp_bcast(int v) {
-  if constexpr (SP == 4) {
-    return __builtin_amdgcn_update_dpp(0, v, J | (J << 2) | (J << 4) | (J << 6), 0xF, 0xF, false);
+// Broadcast of lane J of each SP-lane row group to the whole group, for 32-bit values and
+// doubles (one 64-bit DPP move): DPP quad_perm (SP 4), DPP row_newbcast within 16-lane rows
+// (SP 8, 16), v_readlane (SP 32, 64). Every lane of a group is active at the call (groups are
+// whole rows). The DPP forms run with full row and bank masks and write every lane, so their
+// `old` operand is never read and is left undefined (no move to initialise it).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wuninitialized"
+template <int SP, int J, class V>
+__device__ inline V grp_bcast(V v) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "32- or 64-bit values");
+  if constexpr (SP == 4 || SP == 16) {
+    constexpr int ctl = SP == 4 ? (J | (J << 2) | (J << 4) | (J << 6)) : 0x150 + J;
+    V old;
+    return __builtin_amdgcn_update_dpp(old, v, ctl, 0xF, 0xF, false);
   } else if constexpr (SP == 8) {
-    const int lo = __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, v, 0x150 + 8 + J, 0xF, 0xF, false);
+    V old;
+    const V lo = __builtin_amdgcn_update_dpp(old, v, 0x150 + J, 0xF, 0xF, false);
+    const V hi = __builtin_amdgcn_update_dpp(old, v, 0x150 + 8 + J, 0xF, 0xF, false);
     return (threadIdx.x & 8) ? hi : lo;
-  } else if constexpr (SP == 16) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+  } else if constexpr (sizeof(V) == 8) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = grp_bcast<SP, J>(int(u & 0xffffffffLL));
+    const int hi = grp_bcast<SP, J>(int(u >> 32));
+    return __builtin_bit_cast(V, ((long long)hi << 32) | (unsigned int)lo);
   } else if constexpr (SP == 32) {
-    const int lo = __builtin_amdgcn_readlane(v, J);
-    const int hi = __builtin_amdgcn_readlane(v, 32 + J);
-    return (threadIdx.x & 32) ? hi : lo;
+    const int lo = __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), J);
+    const int hi = __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32 + J);
+    return __builtin_bit_cast(V, (threadIdx.x & 32) ? hi : lo);
   } else {
     static_assert(SP == 64, "row groups of 4..64 lanes");
-    return __builtin_amdgcn_readlane(v, J);
+    return __builtin_bit_cast(V, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), J));
   }
 }
-
-template <int SP, int J>
-__device__ inline double grp_bcast_d(double v) {
-  const long long u = __builtin_bit_cast(long long, v);
-  const int lo = grp_bcast<SP, J>(int(u & 0xffffffffLL));
-  const int hi = grp_bcast<SP, J>(int(u >> 32));
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
+#pragma clang diagnostic pop
 
 template <int J, int N, class F>
 __device__ inline void static_for(F&& f) {
@@ -458,58 +463,111 @@ __device__ inline void static_for(F&& f) {
   }
 }
 
-// One chunk of up to SP entries of a row: lane l of the group holds entry k0 + l (b = 0 for a
-// real entry); the entries are broadcast one at a time and every lane gathers its system's x.
-// Full chunks run branch-free so that all SP gathers are in flight together; lanes of retired
-// systems read x[s] (row 0, cache-resident) instead of branching around the gather.
-template <class T, int SP>
-__device__ inline void rg_chunk(int cnt, int c, double a, double bb, double w, bool run,
-                                const typename Vec2<T>::type* __restrict__ x, int s, double& yr,
-                                double& yi) {
-  using T2 = typename Vec2<T>::type;
-  auto step = [&](auto J) {
-    const int cj = grp_bcast<SP, J>(c);
-    const double aj = grp_bcast_d<SP, J>(a);
-    const double bj = grp_bcast_d<SP, J>(bb) * w;
-    const size_t xi = run ? (size_t)cj * SP + s : (size_t)s;
-    const T2 xv = x[xi];
+// One chunk of up to SP entries of a row. Lane l of the group holds entry k0 + l: the byte
+// offset of its column's SP x values (`off`), a, and b (0 for real entries); the first `nc`
+// entries of the chunk are complex. The entries are broadcast one at a time and every lane
+// gathers its own system's x through `gather(offset)`. A full chunk with no complex entry
+// skips the b terms (a group-uniform branch taken per chunk, so that inside a chunk all SP
+// gathers are in flight together); per entry a lane then spends one 32-bit and one 64-bit
+// broadcast, the gather and two FMAs, and a complex chunk another broadcast, a multiply and
+// two FMAs. Gathers are issued BATCH at a time (a scheduling barrier between batches), which
+// bounds the bytes a wave has in flight.
+template <class T, int SP, int BATCH, class Gather>
+__device__ inline void row_chunk(int cnt, int nc, unsigned off, double a, double bb, double w,
+                                 const Gather& gather, double& yr, double& yi) {
+  static_assert(SP % BATCH == 0, "whole batches");
+  auto step = [&](auto J, auto cplx) {
+    const auto xv = gather(grp_bcast<SP, J>(off));
     const double xr = double(xv.x), xm = double(xv.y);
-    yr += aj * xr - bj * xm;
-    yi += aj * xm + bj * xr;
+    const double aj = grp_bcast<SP, J>(a);
+    yr = __builtin_fma(aj, xr, yr);
+    yi = __builtin_fma(aj, xm, yi);
+    if constexpr (decltype(cplx)::value) {
+      const double bj = grp_bcast<SP, J>(bb) * w;
+      yr = __builtin_fma(-bj, xm, yr);
+      yi = __builtin_fma(bj, xr, yi);
+    }
+  };
+  using Real = std::false_type;
+  using Cplx = std::true_type;
+  auto batches = [&](auto kind) {
+    static_for<0, SP / BATCH>([&](auto B) {
+      if constexpr (B > 0) __builtin_amdgcn_sched_barrier(0);
+      static_for<0, BATCH>([&](auto J) {
+        step(std::integral_constant<int, B * BATCH + J>{}, kind);
+      });
+    });
   };
   if (cnt >= SP) {
-    static_for<0, SP>(step);
+    if (nc <= 0)
+      batches(Real{});
+    else
+      batches(Cplx{});
   } else {
     static_for<0, SP>([&](auto J) {
-      if (J < cnt) step(J);
+      if (J < cnt) step(J, Cplx{});
     });
   }
 }
 
+// x of a row kernel as a raw buffer (byte offsets, 32-bit): loads past num_records return 0
+// without touching memory, which retired systems use to skip their gathers.
+template <class T2>
+__device__ inline T2 buffer_gather(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T2) == 16) {
+    return __builtin_bit_cast(T2, __builtin_amdgcn_raw_buffer_load_b128(r, int(off), 0, 0));
+  } else {
+    static_assert(sizeof(T2) == 8, "float2 or double2");
+    return __builtin_bit_cast(T2, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
+  }
+}
+
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer descriptor word 3 (CDNA)
+// A byte offset past every buffer built by buf_rsrc: raw buffer loads there return 0 and
+// touch no memory, which replaces the branch around a load that must not happen.
+constexpr unsigned kOob = 0xFFFFFF00u;
+
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, long long bytes) {
+  const unsigned n = bytes < (long long)kOob ? unsigned(bytes) : kOob;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(n),
+                                           kBufferDword3);
+}
+__device__ inline unsigned buf_u32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, int(off), 0, 0);
+}
+__device__ inline unsigned buf_u16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, int(off), 0, 0);
+}
+__device__ inline double buf_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
+}
+
 // Matrix data of one row for its lane group: the row's bounds and, per lane, entries beg + s
-// and beg + SP + s (the first two chunks; b = 0 for real entries and past the end).
+// and beg + SP + s (the first two chunks; b = 0 for real entries and past the end), their
+// columns as byte offsets into x (column * SP values).
 struct RgRow {
   int beg, end, bbeg, ncx;
-  int c0, c1;
+  unsigned o0, o1;
   double a0, b0, a1, b1;
 };
 
-template <int SP>
+template <class T, int SP>
 __device__ inline void rg_load_row(const RowView& A, int row, int s, RgRow& R) {
+  constexpr unsigned kStride = SP * sizeof(typename Vec2<T>::type);
   R.beg = A.rp[row];
   R.end = A.rp[row + 1];
   R.bbeg = A.rb[row];
   R.ncx = A.rb[row + 1] - R.bbeg;
-  R.c0 = R.c1 = 0;
+  R.o0 = R.o1 = 0;
   R.a0 = R.b0 = R.a1 = R.b1 = 0.0;
   const int k0 = R.beg + s, k1 = R.beg + SP + s;
   if (k0 < R.end) {
-    R.c0 = A.col[k0];
+    R.o0 = unsigned(A.col[k0]) * kStride;
     R.a0 = A.a[k0];
     if (s < R.ncx) R.b0 = A.b[R.bbeg + s];
   }
   if (k1 < R.end) {
-    R.c1 = A.col[k1];
+    R.o1 = unsigned(A.col[k1]) * kStride;
     R.a1 = A.a[k1];
     if (SP + s < R.ncx) R.b1 = A.b[R.bbeg + SP + s];
   }
@@ -520,8 +578,11 @@ __device__ inline void rg_load_row(const RowView& A, int row, int s, RgRow& R) {
 // matrix costs one load per entry per group instead of one per entry per lane. The first two
 // chunks of a row (32 entries at SP = 16: a whole interior row of a hexahedral mesh) are
 // loaded together, and the next tile's are loaded while the current tile's gathers are in
-// flight, so that the dependent row-pointer -> entry -> gather chains overlap. Same modes,
-// outputs and partials as k_spmv; the diagonal is applied first.
+// flight, so that the dependent row-pointer -> entry -> gather chains overlap. The gathers use
+// 32-bit byte offsets from x (the host keeps x under 4 GiB for this kernel) and are issued four
+// at a time: with all 16 of a chunk in flight at once the FP64 kernel ran at half the speed
+// (0.91 -> 1.84 ms, 16 systems on 1.84 M rows). Same modes, outputs and partials as k_spmv;
+// the diagonal is applied first.
 template <class T, int SP, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* omega,
                                                     const typename Vec2<T>::type* __restrict__ x,
@@ -542,6 +603,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* ome
   const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP
   const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
   const double w = omega[s];
+  const unsigned char* xb = reinterpret_cast<const unsigned char*>(x);
+  const unsigned s_off = unsigned(s * sizeof(T2));
+  auto gather = [&](unsigned o) { return *reinterpret_cast<const T2*>(xb + (o + s_off)); };
   // N is a multiple of SP: whole groups are in or out of range together
   auto row_of = [&](long long tile) { return tile * kBlock + threadIdx.x < N
                                           ? int((tile * kBlock + threadIdx.x) / SP) : -1; };
@@ -549,14 +613,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* ome
   long long tile = tw.first;
   if (tile < tw.last) {
     const int r0 = row_of(tile);
-    if (r0 >= 0) rg_load_row<SP>(A, r0, s, cur);
+    if (r0 >= 0) rg_load_row<T, SP>(A, r0, s, cur);
   }
   for (; tile < tw.last; tile += tw.step) {
     const int row = row_of(tile);
     const long long ntile = tile + tw.step;
     if (ntile < tw.last) {
       const int nrow = row_of(ntile);
-      if (nrow >= 0) rg_load_row<SP>(A, nrow, s, nxt);
+      if (nrow >= 0) rg_load_row<T, SP>(A, nrow, s, nxt);
     }
     if (row >= 0) {
       const long long t = (long long)row * SP + s;
@@ -565,50 +629,27 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rg(RowView A, const double* ome
       double yr = dA * double(xs.x) - dB * double(xs.y);
       double yi = dA * double(xs.y) + dB * double(xs.x);
       const int len = cur.end - cur.beg;
-      rg_chunk<T, SP>(len, cur.c0, cur.a0, cur.b0, w, run, x, s, yr, yi);
-      if (len > SP) rg_chunk<T, SP>(len - SP, cur.c1, cur.a1, cur.b1, w, run, x, s, yr, yi);
+      row_chunk<T, SP, 4>(len, cur.ncx, cur.o0, cur.a0, cur.b0, w, gather, yr, yi);
+      if (len > SP)
+        row_chunk<T, SP, 4>(len - SP, cur.ncx - SP, cur.o1, cur.a1, cur.b1, w, gather, yr,
+                             yi);
       for (int k0 = cur.beg + 2 * SP; k0 < cur.end; k0 += SP) {  // rows longer than 2 chunks
         const int l = k0 - cur.beg + s;
-        int c = 0;
+        unsigned o = 0;
         double a = 0.0, bb = 0.0;
         if (k0 + s < cur.end) {
-          c = A.col[k0 + s];
+          o = unsigned(A.col[k0 + s]) * unsigned(SP * sizeof(T2));
           a = A.a[k0 + s];
           if (l < cur.ncx) bb = A.b[cur.bbeg + l];
         }
-        rg_chunk<T, SP>(cur.end - k0, c, a, bb, w, run, x, s, yr, yi);
+        row_chunk<T, SP, 4>(cur.end - k0, cur.ncx - (k0 - cur.beg), o, a, bb, w, gather,
+                             yr, yi);
       }
       if (run) spmv_store<T, MODE>(t, yr, yi, x, t, y, b, ym, sm, v);
     }
     cur = nxt;
   }
   block_partials<SP, 2>(v, partials);
-}
-
-// rg_chunk with x read from the block's LDS copy (lc = the entry's halo position). A helper of
-// its own: sharing rg_chunk between a global and an LDS source changes how the compiler
-// schedules k_spmv_rg (100 -> 183 VGPRs at 16 systems).
-template <class T, int SP>
-__device__ inline void lds_chunk(int cnt, int lc, double a, double bb, double w,
-                                 const typename Vec2<T>::type* xs, int s, double& yr,
-                                 double& yi) {
-  using T2 = typename Vec2<T>::type;
-  auto step = [&](auto J) {
-    const int cj = grp_bcast<SP, J>(lc);
-    const double aj = grp_bcast_d<SP, J>(a);
-    const double bj = grp_bcast_d<SP, J>(bb) * w;
-    const T2 xv = xs[cj * SP + s];
-    const double xr = double(xv.x), xm = double(xv.y);
-    yr += aj * xr - bj * xm;
-    yi += aj * xm + bj * xr;
-  };
-  if (cnt >= SP) {
-    static_for<0, SP>(step);
-  } else {
-    static_for<0, SP>([&](auto J) {
-      if (J < cnt) step(J);
-    });
-  }
 }
 
 // Halo-staged SpMV over 16-row blocks (batch widths 4..16). A workgroup copies the x values
@@ -641,14 +682,30 @@ __global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const 
   const int sub = int(threadIdx.x / BT);
   const int rl = int(threadIdx.x % BT) / SP;     // row within the block
   T2* xs = reinterpret_cast<T2*>(blk_lds) + (size_t)sub * hmax * SP;
+  const unsigned char* xs_s = reinterpret_cast<const unsigned char*>(xs + s);
+  auto gather = [&](unsigned o) { return *reinterpret_cast<const T2*>(xs_s + o); };
+  constexpr unsigned kStride = SP * sizeof(T2);
   const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
   const double w = omega[s];
   double v[2] = {0.0, 0.0};
   const long long ntiles = ((long long)A.nblk + BW - 1) / BW;
   TileWalk tw(ntiles);
 
-  // halo slot q of this thread is column rl + 16 q of its block; -1 past the halo
-  int ci[NI];
+  // Every load of the staging and the row data is a raw buffer load with a 32-bit offset;
+  // loads that must not happen (past the halo, past the row, retired systems) use offset kOob
+  // and return 0. The row pack's entry arrays are addressed from the tile's first entry.
+  const __amdgpu_buffer_rsrc_t hr = buf_rsrc(A.halo, A.nhalo * 4);
+  const __amdgpu_buffer_rsrc_t xr = buf_rsrc(x, (long long)A.nb * SP * (long long)sizeof(T2));
+  const __amdgpu_buffer_rsrc_t rpr = buf_rsrc(A.rp, ((long long)A.nb + 1) * 4);
+  const __amdgpu_buffer_rsrc_t rbr = buf_rsrc(A.rb, ((long long)A.nb + 1) * 4);
+  const __amdgpu_buffer_rsrc_t dlr = buf_rsrc(A.ldiag, (long long)A.nb * 2);
+  const __amdgpu_buffer_rsrc_t dar = buf_rsrc(A.da, (long long)A.nb * 8);
+  const __amdgpu_buffer_rsrc_t dbr = buf_rsrc(A.db, (long long)A.nb * 8);
+  const unsigned s_off = unsigned(s * sizeof(T2));
+
+  // halo slot q of this thread is column rl + 16 q of its block (cm: the slots in the halo)
+  unsigned ci[NI];
+  unsigned cm = 0;
   auto load_idx = [&](long long tile) {
     const long long blk = tile * BW + sub;
     int h0 = 0, hn = 0;
@@ -656,63 +713,60 @@ __global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const 
       h0 = A.hoff[blk];
       hn = A.hoff[blk + 1] - h0;
     }
+    cm = 0;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
       const int i = rl + q * kRowBlock;
-      ci[q] = i < hn ? A.halo[h0 + i] : -1;
+      ci[q] = buf_u32(hr, i < hn ? unsigned(h0 + i) * 4u : kOob);
+      cm |= (i < hn ? 1u : 0u) << q;
     }
   };
   T2 xv[NI];
+  unsigned xmask = 0;  // slots of the staged tile that hold a halo column
   auto load_x = [&]() {
+    xmask = cm;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
-      xv[q].x = T(0);
-      xv[q].y = T(0);
-      if (run && ci[q] >= 0) xv[q] = x[(size_t)ci[q] * SP + s];
+      const bool take = run && (cm & (1u << q));
+      xv[q] = buffer_gather<T2>(xr, take ? ci[q] * kStride + s_off : kOob);
     }
   };
-  unsigned int xmask = 0;  // slots of the staged tile that hold a halo column
-  auto mask_of = [&]() {
-    unsigned int m = 0;
-#pragma unroll
-    for (int q = 0; q < NI; ++q) m |= (ci[q] >= 0 ? 1u : 0u) << q;
-    return m;
-  };
   struct Row {
-    int beg, end, bbeg, ncx, dl, c0, c1;
+    int beg, end, bbeg, ncx, dl;
+    unsigned o0, o1;
     double a0, b0, a1, b1, dA, dB;
   };
   auto load_row = [&](long long tile, Row& R) {
-    const long long rowl = (tile * BW + sub) * kRowBlock + rl;
-    R.beg = R.end = R.bbeg = R.ncx = R.dl = R.c0 = R.c1 = 0;
-    R.a0 = R.b0 = R.a1 = R.b1 = R.dA = R.dB = 0.0;
-    if (rowl < A.nb) {
-      const int row = int(rowl);
-      R.beg = A.rp[row];
-      R.end = A.rp[row + 1];
-      R.bbeg = A.rb[row];
-      R.ncx = A.rb[row + 1] - R.bbeg;
-      R.dl = A.ldiag[row];
-      R.dA = A.da[row];
-      R.dB = A.db[row] * w;
-      if (R.beg + s < R.end) {
-        R.c0 = A.lcol[R.beg + s];
-        R.a0 = A.a[R.beg + s];
-        if (s < R.ncx) R.b0 = A.b[R.bbeg + s];
-      }
-      if (R.beg + SP + s < R.end) {
-        R.c1 = A.lcol[R.beg + SP + s];
-        R.a1 = A.a[R.beg + SP + s];
-        if (SP + s < R.ncx) R.b1 = A.b[R.bbeg + SP + s];
-      }
-    }
+    const long long r0 = tile * BW * kRowBlock;  // the tile's first row (uniform, < nb)
+    const long long rowl = r0 + sub * kRowBlock + rl;
+    const unsigned row = unsigned(rowl);
+    const bool ok = rowl < A.nb;
+    const int e0 = A.rp[r0], f0 = A.rb[r0];
+    const __amdgpu_buffer_rsrc_t ar = buf_rsrc(A.a + e0, (A.nnz - e0) * 8);
+    const __amdgpu_buffer_rsrc_t lr = buf_rsrc(A.lcol + e0, (A.nnz - e0) * 2);
+    const __amdgpu_buffer_rsrc_t br = buf_rsrc(A.b + f0, (A.nnzb - f0) * 8);
+    R.beg = int(buf_u32(rpr, ok ? row * 4u : kOob));
+    R.end = int(buf_u32(rpr, ok ? row * 4u + 4u : kOob));
+    R.bbeg = int(buf_u32(rbr, ok ? row * 4u : kOob));
+    R.ncx = int(buf_u32(rbr, ok ? row * 4u + 4u : kOob)) - R.bbeg;
+    R.dl = int(buf_u16(dlr, ok ? row * 2u : kOob));
+    R.dA = buf_f64(dar, ok ? row * 8u : kOob);
+    R.dB = buf_f64(dbr, ok ? row * 8u : kOob) * w;
+    const int len = R.end - R.beg;
+    const unsigned k0 = unsigned(R.beg - e0 + s), k1 = k0 + SP;
+    const unsigned q0 = unsigned(R.bbeg - f0 + s), q1 = q0 + SP;
+    R.o0 = buf_u16(lr, s < len ? k0 * 2u : kOob) * kStride;
+    R.a0 = buf_f64(ar, s < len ? k0 * 8u : kOob);
+    R.b0 = buf_f64(br, s < R.ncx ? q0 * 8u : kOob);
+    R.o1 = buf_u16(lr, SP + s < len ? k1 * 2u : kOob) * kStride;
+    R.a1 = buf_f64(ar, SP + s < len ? k1 * 8u : kOob);
+    R.b1 = buf_f64(br, SP + s < R.ncx ? q1 * 8u : kOob);
   };
 
   Row cur, nxt;
   long long tile = tw.first;
   if (tile < tw.last) {
     load_idx(tile);
-    xmask = mask_of();
     load_x();
     load_row(tile, cur);
     if (tile + tw.step < tw.last) load_idx(tile + tw.step);
@@ -724,7 +778,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const 
     __syncthreads();
     const long long nt = tile + tw.step;
     if (nt < tw.last) {  // ci holds tile nt's halo
-      xmask = mask_of();
       load_x();
       load_row(nt, nxt);
       if (nt + tw.step < tw.last) load_idx(nt + tw.step);
@@ -736,18 +789,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const 
       double yr = cur.dA * double(xt.x) - cur.dB * double(xt.y);
       double yi = cur.dA * double(xt.y) + cur.dB * double(xt.x);
       const int len = cur.end - cur.beg;
-      lds_chunk<T, SP>(len, cur.c0, cur.a0, cur.b0, w, xs, s, yr, yi);
-      if (len > SP) lds_chunk<T, SP>(len - SP, cur.c1, cur.a1, cur.b1, w, xs, s, yr, yi);
+      row_chunk<T, SP, SP>(len, cur.ncx, cur.o0, cur.a0, cur.b0, w, gather, yr, yi);
+      if (len > SP)
+        row_chunk<T, SP, SP>(len - SP, cur.ncx - SP, cur.o1, cur.a1, cur.b1, w, gather, yr,
+                             yi);
       for (int k0 = cur.beg + 2 * SP; k0 < cur.end; k0 += SP) {  // rows longer than 2 chunks
         const int l = k0 - cur.beg + s;
-        int c = 0;
+        unsigned o = 0;
         double a = 0.0, bb = 0.0;
         if (k0 + s < cur.end) {
-          c = A.lcol[k0 + s];
+          o = unsigned(A.lcol[k0 + s]) * kStride;
           a = A.a[k0 + s];
           if (l < cur.ncx) bb = A.b[cur.bbeg + l];
         }
-        lds_chunk<T, SP>(cur.end - k0, c, a, bb, w, xs, s, yr, yi);
+        row_chunk<T, SP, SP>(cur.end - k0, cur.ncx - (k0 - cur.beg), o, a, bb, w, gather,
+                             yr, yi);
       }
       if (run) spmv_store<T, MODE>(t, yr, yi, xs, cur.dl * SP + s, y, b, ym, sm, v);
     }
@@ -1478,6 +1534,9 @@ BlkView blk_view(mtcocr_handle* h) {
   v.db = h->db64.as<double>();
   v.nb = h->nb;
   v.nblk = h->nblk;
+  v.nhalo = (long long)(h->halo.bytes / sizeof(int));
+  v.nnz = (long long)(h->pa.bytes / sizeof(double));
+  v.nnzb = (long long)(h->pb.bytes / sizeof(double));
   return v;
 }
 
@@ -1489,6 +1548,12 @@ int blk_lds_bytes(const mtcocr_handle* h) {
   return kRowBlock * h->halo_max * int(sizeof(typename Vec2<TV>::type));
 }
 
+// The row kernels address x with 32-bit byte offsets: batch vectors of nb x SP values of
+// `elem` bytes must stay under 4 GiB (else the sliced ELL runs)
+inline bool rows_fit(const mtcocr_handle* h, int sp, int elem) {
+  return (long long)h->nb * sp * elem <= (long long)kOob;
+}
+
 // SpMV launch in the layout chosen for this batch width (vector type TV, which the outer FP64
 // residual of the mixed solver uses with an FP32 engine).
 template <class TV, int SP, int MODE>
@@ -1497,6 +1562,7 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  const typename Vec2<TV>::type* b, typename Vec2<TV>::type* ym,
                  typename Vec2<TV>::type* sm, const SysScalars* sc, const int* nrun,
                  double* part) {
+  if (!rows_fit(h, SP, int(sizeof(typename Vec2<TV>::type)))) layout = 0;
   if constexpr (SP >= 4 && SP <= 16) {
     const int lds = blk_lds_bytes<TV>(h);
     if (layout == 2 && lds > 0) {
@@ -2453,6 +2519,7 @@ namespace {
 int effective_layout(const mtcocr_handle* h, int q, int k) {
   int l = h->spmv_layout[q][k];
   if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
+  if (!rows_fit(h, 1 << k, q == 0 ? 16 : 8)) l = 0;
   return l;
 }
 }  // namespace
@@ -2519,7 +2586,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
       }
       for (int fam = 0; fam < 3; ++fam) {
         const int nlayouts =
-            (fam != 0 || SP < 4) ? 1 : (SP <= 16 && blk_lds_bytes<T>(h) > 0) ? 3 : 2;
+            (fam != 0 || SP < 4 || !rows_fit(h, SP, int(sizeof(T2)))) ? 1
+            : (SP <= 16 && blk_lds_bytes<T>(h) > 0)                  ? 3
+                                                                      : 2;
         for (int layout = 0; layout < nlayouts; ++layout) {
           // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
           // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
