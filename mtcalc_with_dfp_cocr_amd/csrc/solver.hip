@@ -351,10 +351,8 @@ __device__ inline void spmv_store(long long t, double yr, double yi,
     y[t] = o;
     v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
     v[1] += double(o.x) * double(o.x) + double(o.y) * double(o.y);
-    if (ym) {
-      ym[t] = xsrc[tx];
-      sm[t] = o;
-    }
+    if (ym) ym[t] = xsrc[tx];
+    if (sm) sm[t] = o;
   }
 }
 
@@ -937,7 +935,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
 //      and the partials (sm, u), (u, u) of u = r - sm for the MRS weight of k.
 // Step 2 is skipped by a system that halts in iteration k. Deferring the MRS step saves
 // re-reading x and r, and forming p here saves the preconditioner pass reading s and p: 8 vector
-// reads and 6 writes instead of 13 and 6.
+// reads and 6 writes instead of 13 and 6. With ym null (the FP64 solver, which returns x
+// unsmoothed) the ym blend is skipped: 7 reads and 5 writes.
 template <class T, int SP>
 __global__ __launch_bounds__(kBlock) void k_xr_fused(
     const typename Vec2<T>::type* __restrict__ xin, typename Vec2<T>::type* __restrict__ xout,
@@ -962,17 +961,19 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
     const int mode = sc[s].mode;
     if (mode == MRS_BLEND) {
       const T wm = T(sc[s].w);
-      T2 yv = ym[t];
+      if (ym) {
+        T2 yv = ym[t];
+        yv.x += wm * (xv.x - yv.x);
+        yv.y += wm * (xv.y - yv.y);
+        ym[t] = yv;
+      }
       const T ur = rv.x - smv.x, ui = rv.y - smv.y;
-      yv.x += wm * (xv.x - yv.x);
-      yv.y += wm * (xv.y - yv.y);
       smv.x += wm * ur;
       smv.y += wm * ui;
-      ym[t] = yv;
       sm[t] = smv;
       v[2] += double(smv.x) * double(smv.x) + double(smv.y) * double(smv.y);
     } else if (mode == MRS_COPY) {
-      ym[t] = xv;
+      if (ym) ym[t] = xv;
       smv = rv;
       sm[t] = smv;
       v[2] += double(rv.x) * double(rv.x) + double(rv.y) * double(rv.y);
@@ -1649,6 +1650,11 @@ struct Engine {
     layout = h->spmv_layout[q][k];
   }
 
+  // The MRS-smoothed iterate: the FP32 inner solve of the mixed solver returns it as its
+  // correction; the FP64 solver returns x unsmoothed, as the reference does (its MRS keeps y
+  // but nothing reads it back), so there it is not kept (null: no traffic).
+  T2* ym_or_null() const { return std::is_same<T, float>::value ? ym : nullptr; }
+
   // partials of a grid of nblk workgroups, P values each, as the scalar kernels read them:
   // folded by k_fold into part2 first when the grid is wide
   template <int P>
@@ -1677,7 +1683,7 @@ struct Engine {
 
   // stages 1-4 of the request order: r = b - A x, X0 test, s = M^-1 r, p = a = w = 0, z = A s
   void prologue(hipStream_t st, int maxiter) {
-    spmv<1>(st, x, r, b, ym, sm);
+    spmv<1>(st, x, r, b, ym_or_null(), sm);
     int nr;
     const double* pp = partials<SP * 2>(st, grid_s, nr);
     s_prologue<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
@@ -1701,8 +1707,8 @@ struct Engine {
     int nr;
     const double* pp = partials<SP * 2>(st, grid_e, nr);
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    k_xr_fused<T, SP><<<grid_e, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w, ym,
-                                                 sm, sc, nrun, part, nb);
+    k_xr_fused<T, SP><<<grid_e, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w,
+                                                 ym_or_null(), sm, sc, nrun, part, nb);
     pp = partials<SP * 3>(st, grid_e, nr);
     s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
