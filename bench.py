@@ -152,6 +152,10 @@ def cpu_baseline(M, m, ma, case, iters_needed, budget_iters, threads):
     return est, per_iter, setup, stn["seconds"] + st1["seconds"]
 
 
+# SpMV kernel of each layout name the autotuner reports
+SPMV_KERNELS = {"ell": "k_spmv", "rowpack": "k_spmv_rg", "rowblock": "k_spmv_blk"}
+
+
 def measure_traffic(args, sp):
     """HBM bytes per launch of the batched FP64 SpMV: two rocprofv3 --pmc passes (FETCH_SIZE,
     then WRITE_SIZE) over this script's --kernels-only mode, each scaled by what the counter
@@ -167,7 +171,7 @@ def measure_traffic(args, sp):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    pat = re.compile(r"k_spmv(_rg)?<double, %d, 0>" % sp)
+    pat = re.compile(r"k_spmv(_rg|_blk)?<double, %d, 0>" % sp)
     res = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
@@ -449,7 +453,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_bytes,
                      "traffic_counters": traffic,
-                     "kernel": f"k_spmv<double,{spb},0> (batched FP64 SpMV)",
+                     "kernel": f"{SPMV_KERNELS[tune['batched_fp64']['spmv_layout']]}"
+                               f"<double,{spb},0> (batched FP64 SpMV)",
                      "algorithmic_bytes": spmv_bytes, "avg_ms": spmv_ms},
         "cpu_baseline": cpu,
         "fp64_gpu_per_system_s": fp64_per_sys,

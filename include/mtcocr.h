@@ -159,14 +159,19 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
                       double* bytes_per_launch);
 
 /* Launch shapes: workgroup caps of the grid-stride SpMV, edge-streaming and node kernels
- * (multiples of 8 in [8, 16384]; 0 keeps a value) and the SpMV layout (0 sliced ELL, 1 row
- * pack with one row per group of lanes, used for batches of 4..64 systems, 2 row blocks: the
- * row pack over 16-row blocks whose x values are staged in LDS, used for batches of 4..16
- * systems when no block has more than 192 distinct columns; elsewhere the row pack runs in its place; -1 keeps it). set_launch applies to both
- * precisions and every batch width; get_launch reports those of `precision` at the current
- * batch width, the layout as the one that runs. Caps change only the reduction grouping of the dot
- * products and the layout only the summation order inside a row (results are deterministic for
- * a fixed shape). Defaults: caps 8192 / 2048 / 8192, row pack from 16 systems up. */
+ * (multiples of 8 in [8, 16384]; 0 keeps a value) and the SpMV layout (-1 keeps it):
+ *   0 sliced ELL (any batch width);
+ *   1 row pack, one row per group of lanes (batches of 3..64 systems);
+ *   2 row blocks: the row pack over 16-row blocks whose x values are staged in LDS (batches of
+ *     3..16 systems, when no block has more than 192 distinct columns); where it does not
+ *     apply, the row pack runs.
+ * The row layouts need the batch vectors under 4 GiB (nb x width x 16 bytes in FP64); above
+ * that the sliced ELL runs. set_launch applies to both precisions and every batch width;
+ * get_launch reports those of `precision` at the current batch width, the layout as the one
+ * that runs. Caps change only the reduction grouping of the dot products and the layout only
+ * the summation order inside a row (results are deterministic for a fixed shape). Defaults:
+ * caps 8192 / 2048 / 8192, sliced ELL for 1..2 systems, row blocks for 3..16, row pack for
+ * 17..64. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,

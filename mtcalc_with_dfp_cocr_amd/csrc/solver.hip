@@ -1431,9 +1431,8 @@ struct mtcocr_handle {
       }
   }
   mtcocr_handle() {
-    set_caps(8192, 2048, 8192, 0);
-    for (int p = 0; p < 2; ++p)
-      for (int k = 4; k < 7; ++k) spmv_layout[p][k] = 1;  // row pack from 16 systems up
+    // row blocks for 4..16 systems, the row pack for 32 and 64 (set_caps clamps layout 2 there)
+    set_caps(8192, 2048, 8192, 2);
   }
 
   int64_t device_bytes() const {

@@ -271,11 +271,12 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
     with Solver(0) as B:
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(c1_mesh)
-        assert B.get_launch() == {"cap_spmv": 8192, "cap_vec": 2048, "cap_node": 8192,
-                                  "spmv_layout": "ell"}  # 3 systems: batch width 4
+        defaults = {"cap_spmv": 8192, "cap_vec": 2048, "cap_node": 8192}
+        assert B.get_launch() == dict(defaults, spmv_layout="ell")  # no batch yet: width 1
         x0, s0 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        assert B.get_launch() == dict(defaults, spmv_layout="rowblock")  # 3 systems: width 4
         for caps in [(8, 8, 8, "rowpack"), (1024, 512, 16384, "ell"),
-                     (16384, 16384, 1024, "rowpack")]:
+                     (16384, 16384, 1024, "rowpack"), (2048, 0, 0, "rowblock")]:
             B.set_launch(*caps)
             x1, s1 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
             x2, s2 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
