@@ -157,10 +157,11 @@ SPMV_KERNELS = {"ell": "k_spmv", "rowpack": "k_spmv_rg", "rowblock": "k_spmv_blk
 
 
 def measure_traffic(args, sp):
-    """HBM bytes per launch of the batched FP64 SpMV: two rocprofv3 --pmc passes (FETCH_SIZE,
-    then WRITE_SIZE) over this script's --kernels-only mode, each scaled by what the counter
-    reads for the 1 GiB copy kernel k_copy (known read and write bytes). None when rocprofv3
-    is unavailable or a pass fails."""
+    """HBM bytes per launch of the batched FP64 SpMV and of the fused update pass: two
+    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over this script's --kernels-only
+    mode, each scaled by what the counter reads for the 1 GiB copy kernel k_copy (known read and
+    write bytes). Returns {"spmv": {...}, "update": {...}}; None when rocprofv3 is unavailable or
+    a pass fails."""
     import csv
     import re
     import shutil
@@ -171,8 +172,9 @@ def measure_traffic(args, sp):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    pat = re.compile(r"k_spmv(_rg|_blk)?<double, %d, 0>" % sp)
-    res = {}
+    pats = {"spmv": re.compile(r"k_spmv(_rg|_blk)?<double, %d, 0>" % sp),
+            "update": re.compile(r"k_xr_fused<double, %d>" % sp)}
+    res = {"spmv": {}, "update": {}}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--output-format", "csv", "--",
@@ -198,17 +200,20 @@ def measure_traffic(args, sp):
             log(f"traffic pass {ctr} failed (rc={rc})")
             return None
         rows = sorted(csv.DictReader(open(files[0])), key=lambda r: int(r["Dispatch_Id"]))
-        spmv = [float(r["Counter_Value"]) for r in rows if pat.search(r["Kernel_Name"])
-                and r["Counter_Name"] == ctr]
         copy = [float(r["Counter_Value"]) for r in rows if "k_copy" in r["Kernel_Name"]
                 and r["Counter_Name"] == ctr]
         shutil.rmtree(d, ignore_errors=True)
-        if len(spmv) < 5 or not copy:
+        if not copy:
             return None
-        kib = float(np.mean(spmv[-5:]))           # the timed launches at the tuned shape
         factor = (1 << 30) / (float(np.mean(copy)) * 1024.0)
-        res[ctr] = {"kib_per_dispatch": kib, "calibration": factor,
-                    "bytes": kib * 1024.0 * factor}
+        for name, pat in pats.items():
+            vals = [float(r["Counter_Value"]) for r in rows if pat.search(r["Kernel_Name"])
+                    and r["Counter_Name"] == ctr]
+            if len(vals) < 5:
+                return None
+            kib = float(np.mean(vals[-5:]))           # the timed launches at the tuned shape
+            res[name][ctr] = {"kib_per_dispatch": kib, "calibration": factor,
+                              "bytes": kib * 1024.0 * factor}
     return res
 
 
@@ -317,9 +322,12 @@ def main():
     if args.kernels_only:
         S.set_rhs(prs, om)
         ms, by = S.bench_spmv("fp64", args.spmv_reps)
+        ums, uby = S.bench_update("fp64", args.spmv_reps)
         cms = S.bench_copy(1 << 30, args.spmv_reps)
         print(json.dumps({"spmv_fp64_batched_ms": ms, "spmv_algorithmic_bytes": by,
-                          "spmv_gbs": by / (ms * 1e-3) / 1e9, "copy_bytes": 1 << 30,
+                          "spmv_gbs": by / (ms * 1e-3) / 1e9,
+                          "update_fp64_batched_ms": ums, "update_algorithmic_bytes": uby,
+                          "update_gbs": uby / (ums * 1e-3) / 1e9, "copy_bytes": 1 << 30,
                           "copy_ms": cms, "copy_gbs": 2 * (1 << 30) / (cms * 1e-3) / 1e9,
                           "launch": tune}), flush=True)
         S.close()
@@ -387,6 +395,10 @@ def main():
     ach, ach32, ach64 = gbs(spmv_bytes, spmv_ms), gbs(spmv32_bytes, spmv32_ms), gbs(spmv64_bytes, spmv64_ms)
     log(f"spmv fp64 x{nsys}: {spmv_ms:.3f} ms {ach:.0f} GB/s; fp32-vector x{nsys}: "
         f"{spmv32_ms:.3f} ms {ach32:.0f} GB/s; fp64 x1: {spmv64_ms:.3f} ms {ach64:.0f} GB/s")
+    S.set_rhs(prs, om)
+    upd_ms, upd_bytes = S.bench_update("fp64", args.spmv_reps)
+    achu = gbs(upd_bytes, upd_ms)
+    log(f"update pass fp64 x{nsys}: {upd_ms:.3f} ms {achu:.0f} GB/s")
     S.close()
 
     # ---- impedance agreement at a 5x5 surface grid (200 m spacing) ---------------------------
@@ -407,6 +419,7 @@ def main():
             AD.main(["--fd-cells", str(args.adjoint_cells)])
         adj = json.loads(buf.getvalue().strip().splitlines()[-1])
         adj = {k: adj[k] for k in ("workload", "J", "eps", "h", "rel_err_norm", "rel_err",
+                                   "rel_err_max", "directions",
                                    "seconds", "timings")}
         log(f"adjoint (config 4): {adj} ({time.perf_counter() - t0:.1f}s)")
 
@@ -429,8 +442,20 @@ def main():
     spb = 1 << (nsys - 1).bit_length()
     traffic = None if args.no_traffic else measure_traffic(args, spb)
     log(f"spmv traffic: {traffic}")
-    traffic_bytes = (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]
-                     if traffic else None)
+    def kernel_line(kname, b, ms, tr):
+        a = gbs(b, ms)
+        t = tr["FETCH_SIZE"]["bytes"] + tr["WRITE_SIZE"]["bytes"] if tr else None
+        return {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": a / HBM_PEAK_GBS, "traffic": t, "traffic_counters": tr,
+                "kernel": kname, "algorithmic_bytes": b, "avg_ms": ms}
+
+    spmv_line = kernel_line(f"{SPMV_KERNELS[tune['batched_fp64']['spmv_layout']]}"
+                            f"<double,{spb},0> (batched FP64 SpMV)", spmv_bytes, spmv_ms,
+                            traffic["spmv"] if traffic else None)
+    upd_line = kernel_line(f"k_xr_fused<double,{spb}> (batched FP64 update pass)", upd_bytes,
+                           upd_ms, traffic["update"] if traffic else None)
+    # the dominant kernel: both run once per iteration, the longer one
+    roof = upd_line if upd_ms > spmv_ms else spmv_line
     out = {
         "metric": METRIC,
         "value": value,
@@ -450,12 +475,9 @@ def main():
                    "systems": nsys, "nb": int(p.nb), "parallelism": "single GPU",
                    "order_block": args.order_block,
                    "launch": tune},
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic_bytes,
-                     "traffic_counters": traffic,
-                     "kernel": f"{SPMV_KERNELS[tune['batched_fp64']['spmv_layout']]}"
-                               f"<double,{spb},0> (batched FP64 SpMV)",
-                     "algorithmic_bytes": spmv_bytes, "avg_ms": spmv_ms},
+        "roofline": roof,
+        "spmv_fp64_batched": spmv_line,
+        "update_fp64_batched": upd_line,
         "cpu_baseline": cpu,
         "fp64_gpu_per_system_s": fp64_per_sys,
         "speedup_vs_fp64_gpu": (fp64_per_sys / value) if fp64_per_sys else None,

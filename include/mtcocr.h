@@ -159,7 +159,7 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
                       double* bytes_per_launch);
 
 /* Launch shapes: workgroup caps of the grid-stride SpMV, edge-streaming and node kernels
- * (multiples of 8 in [8, 16384]; 0 keeps a value) and the SpMV layout (-1 keeps it):
+ * (multiples of 8 in [8, 131072]; 0 keeps a value) and the SpMV layout (-1 keeps it):
  *   0 sliced ELL (any batch width);
  *   1 row pack, one row per group of lanes (batches of 3..64 systems);
  *   2 row blocks: the row pack over 16-row blocks whose x values are staged in LDS (batches of
@@ -170,7 +170,7 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
  * get_launch reports those of `precision` at the current batch width, the layout as the one
  * that runs. Caps change only the reduction grouping of the dot products and the layout only
  * the summation order inside a row (results are deterministic for a fixed shape). Defaults:
- * caps 8192 / 2048 / 8192, sliced ELL for 1..2 systems, row blocks for 3..16, row pack for
+ * caps 8192 / 65536 / 2048, sliced ELL for 1..2 systems, row blocks for 3..16, row pack for
  * 17..64. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
@@ -179,12 +179,22 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 
 /* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),
- * node and edge kernels at caps 512..16384, `reps` launches each, in `precision`, and keeps the
+ * node and edge kernels at caps 512..131072, `reps` launches each, in `precision`, and keeps the
  * fastest choice of each family on the handle for this precision and batch width (padded system
  * count). Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the
  * chosen shape. Writes only work buffers: resident right-hand sides and solutions stay. */
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms);
+
+/* Kernel-level timing hook, no counterpart in the reference (its per-iteration vector updates,
+ * pcocr_rci.cpp:135-154 and MRS.cpp:85-153, run as separate loops): average time (ms) of the
+ * fused update pass (k_xr_fused: the MRS blend, p = s + beta p, x += alpha p, r -= alpha a,
+ * s -= alpha w and their partials) over the systems of the current batch, `reps` launches on
+ * filled work vectors, and the algorithmic bytes per launch (each stream read or written once
+ * per system and edge: 12 streams in FP64, 14 with the FP32 vectors of the mixed solver, which
+ * also smooth x). */
+int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_ms,
+                        double* bytes_per_launch);
 
 /* Calibration hook: average time (ms) of a plain device-to-device copy kernel (k_copy, 16-byte
  * elements) over nbytes (a multiple of 16), `reps` launches after one warm-up. Its HBM byte

@@ -134,6 +134,13 @@ class Solver:
                                         C.byref(ms), C.byref(by)), "mtcocr_bench_spmv")
         return ms.value, by.value
 
+    def bench_update(self, precision: str = "fp64", reps: int = 20):
+        """Average ms and algorithmic bytes of the fused per-iteration update pass."""
+        ms, by = C.c_double(), C.c_double()
+        check(self._L.mtcocr_bench_update(self._h, PRECISION[precision], int(reps),
+                                          C.byref(ms), C.byref(by)), "mtcocr_bench_update")
+        return ms.value, by.value
+
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         """Average ms of the plain copy kernel over nbytes (HBM counter calibration)."""
         ms = C.c_double()

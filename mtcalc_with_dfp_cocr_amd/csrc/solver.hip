@@ -42,7 +42,7 @@ namespace mtcocr {
 // small helpers
 
 constexpr int kBlock = 256;      // threads per block of the streaming kernels
-constexpr int kMaxBlocks = 16384; // largest grid of a reduction kernel (size of the partials)
+constexpr int kMaxBlocks = 131072; // largest grid of a reduction kernel (size of the partials)
 constexpr int kScalarThreads = 1024;
 constexpr int kXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
 
@@ -52,6 +52,28 @@ enum : int { MRS_SKIP = 0, MRS_COPY = 1, MRS_BLEND = 2, MRS_NONE = 3 };
 template <class T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
+
+// Streaming loads and stores with the nontemporal hint, for vectors that a kernel touches once
+// (they are far larger than L2 and the MALL, so caching them only evicts the gathered data).
+template <class T2>
+__device__ inline T2 ld_nt(const T2* p) {
+  using T = decltype(T2::x);
+  using V = T __attribute__((ext_vector_type(2)));
+  const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+  T2 r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+template <class T2>
+__device__ inline void st_nt(T2* p, T2 o) {
+  using T = decltype(T2::x);
+  using V = T __attribute__((ext_vector_type(2)));
+  V v;
+  v.x = o.x;
+  v.y = o.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+}
 
 // Per-system scalar state (double precision in every mode).
 struct SysScalars {
@@ -174,7 +196,7 @@ __device__ inline void reduce_partials(const double* part, int nblk, double* tot
   const int q = t % P, j = t / P;
   double acc = 0.0;
   if (j < G) {
-    // unrolled so that the loads of one thread are in flight together (up to 16384 blocks)
+    // unrolled so that the loads of one thread are in flight together
 #pragma unroll 8
     for (int b = j; b < nblk; b += G) acc += part[(size_t)b * P + q];
   }
@@ -378,14 +400,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(EllView<double> A, const double
   const long long ntiles = (N + kBlock - 1) / kBlock;
   double v[2] = {0.0, 0.0};
   TileWalk tw(ntiles);
-  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+  const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP
+  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
+  const double w = omega[s];
+  for (long long tile = tw.first; run && tile < tw.last; tile += tw.step) {
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
-    const int s = int(t % SP);
-    if ((MODE == 2 ? sc[s].outer_status : sc[s].status) != ST_RUNNING) continue;
     const int row = int(t / SP);
     const int slice = row >> 6, lane = row & 63;
-    const double w = omega[s];
     double yr = 0, yi = 0;
     {
       const int oc = A.offc[slice], ncs = A.nc[slice];
@@ -825,13 +847,14 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
   const long long N = (long long)G.m * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
   TileWalk tw(ntiles);
+  // kBlock is a multiple of SP: a thread serves one system in every tile
+  const int s = int(threadIdx.x % SP);
+  if (sc[s].status != ST_RUNNING) return;
+  const double br = USE_BETA ? sc[s].be_re : 0.0, bi = USE_BETA ? sc[s].be_im : 0.0;
   for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
-    const int s = int(t % SP);
-    if (sc[s].status != ST_RUNNING) continue;
     const int i = int(t / SP);
-    const double br = USE_BETA ? sc[s].be_re : 0.0, bi = USE_BETA ? sc[s].be_im : 0.0;
     T hr = 0, hi = 0;
     for (int k = 0; k < G.wg; ++k) {
       const int e = G.gcol[(size_t)k * G.m + i];
@@ -860,11 +883,11 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
   const long long N = (long long)G.n * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
   TileWalk tw(ntiles);
+  const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP
+  if (sc[s].status != ST_RUNNING) return;
   for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
-    const int s = int(t % SP);
-    if (sc[s].status != ST_RUNNING) continue;
     const int e = int(t / SP);
     const T2 xv = v[t], d = diag_inv<T>(da, e, s);
     T yr = xv.x * d.x - xv.y * d.y;
@@ -897,14 +920,13 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
   const long long ntiles = (N + kBlock - 1) / kBlock;
   double v[2] = {0.0, 0.0};
   TileWalk tw(ntiles);
-  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
-    const long long t = tile * kBlock + threadIdx.x;
-    if (t >= N) continue;
-    const int s = int(t % SP);
-    if (sc[s].status != ST_RUNNING) continue;
+  const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP
+  const bool run = sc[s].status == ST_RUNNING;
+  const double br = sc[s].be_re, bi = sc[s].be_im;
+  // two elements per thread and step, streams nontemporal (help is gathered and stays cached)
+  auto elem = [&](long long t, T2 zv, T2 av) {
     const int e = int(t / SP);
-    const double br = sc[s].be_re, bi = sc[s].be_im;
-    const T2 an = caxpy(z[t], a[t], br, bi);
+    const T2 an = caxpy(zv, av, br, bi);
     const T2 d = diag_inv<T>(da, e, s);
     T yr = an.x * d.x - an.y * d.y;
     T yi = an.x * d.y + an.y * d.x;
@@ -918,10 +940,27 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     T2 o;
     o.x = yr;
     o.y = yi;
-    a[t] = an;
-    w[t] = o;
+    st_nt(a + t, an);
+    st_nt(w + t, o);
     v[0] += double(an.x) * double(yr) - double(an.y) * double(yi);
     v[1] += double(an.x) * double(yi) + double(an.y) * double(yr);
+  };
+  for (long long tile = tw.first; run && tile < tw.last; tile += 2 * tw.step) {
+    const long long t0 = tile * kBlock + threadIdx.x;
+    const long long t1 = (tile + tw.step) * kBlock + threadIdx.x;
+    const bool ok0 = t0 < N;
+    const bool ok1 = tile + tw.step < tw.last && t1 < N;
+    T2 z0, a0, z1, a1;
+    if (ok0) {
+      z0 = ld_nt(z + t0);
+      a0 = ld_nt(a + t0);
+    }
+    if (ok1) {
+      z1 = ld_nt(z + t1);
+      a1 = ld_nt(a + t1);
+    }
+    if (ok0) elem(t0, z0, a0);
+    if (ok1) elem(t1, z1, a1);
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -951,46 +990,73 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
   const long long ntiles = (N + kBlock - 1) / kBlock;
   double v[3] = {0.0, 0.0, 0.0};
   TileWalk tw(ntiles);
-  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
-    const long long t = tile * kBlock + threadIdx.x;
-    if (t >= N) continue;
-    const int s = int(t % SP);
-    if (sc[s].status != ST_RUNNING) continue;
-    const T2 xv = xin[t], rv = r[t];
-    T2 smv = sm[t];
-    const int mode = sc[s].mode;
+  // kBlock is a multiple of SP: a thread serves one system in every tile, so the system's
+  // scalars are read once and every branch below is the same for all of the thread's tiles
+  const int s = int(threadIdx.x % SP);
+  const bool run = sc[s].status == ST_RUNNING;
+  const int mode = sc[s].mode;
+  const T wm = T(sc[s].w);
+  const bool step2 = !sc[s].halt;
+  const double ar = sc[s].al_re, ai = sc[s].al_im, br = sc[s].be_re, bi = sc[s].be_im;
+  // Two elements per thread and step (tiles tile and tile + step), every load of both issued
+  // before any use, nontemporal loads and stores: the MRS pass runs 12 streams at once, and
+  // this form keeps more bytes in flight per wave.
+  struct Elem {
+    T2 x, r, sm, s, p, a, w;
+  };
+  auto load = [&](long long t, Elem& E) {
+    E.x = ld_nt(xin + t);
+    E.r = ld_nt(r + t);
+    E.sm = ld_nt(sm + t);
+    if (step2) {
+      E.s = ld_nt(s_vec + t);
+      E.p = ld_nt(p + t);
+      E.a = ld_nt(a + t);
+      E.w = ld_nt(w + t);
+    }
+  };
+  auto update = [&](long long t, Elem& E) {
+    T2 smv = E.sm;
     if (mode == MRS_BLEND) {
-      const T wm = T(sc[s].w);
       if (ym) {
         T2 yv = ym[t];
-        yv.x += wm * (xv.x - yv.x);
-        yv.y += wm * (xv.y - yv.y);
+        yv.x += wm * (E.x.x - yv.x);
+        yv.y += wm * (E.x.y - yv.y);
         ym[t] = yv;
       }
-      const T ur = rv.x - smv.x, ui = rv.y - smv.y;
+      const T ur = E.r.x - smv.x, ui = E.r.y - smv.y;
       smv.x += wm * ur;
       smv.y += wm * ui;
-      sm[t] = smv;
+      st_nt(sm + t, smv);
       v[2] += double(smv.x) * double(smv.x) + double(smv.y) * double(smv.y);
     } else if (mode == MRS_COPY) {
-      if (ym) ym[t] = xv;
-      smv = rv;
-      sm[t] = smv;
-      v[2] += double(rv.x) * double(rv.x) + double(rv.y) * double(rv.y);
+      if (ym) ym[t] = E.x;
+      smv = E.r;
+      st_nt(sm + t, smv);
+      v[2] += double(E.r.x) * double(E.r.x) + double(E.r.y) * double(E.r.y);
     }
-    if (sc[s].halt) continue;
-    const double ar = sc[s].al_re, ai = sc[s].al_im;
-    const T2 sv = s_vec[t];
-    const T2 pn = caxpy(sv, p[t], sc[s].be_re, sc[s].be_im);
-    const T2 xn = caxpy(xv, pn, ar, ai);
-    const T2 rn = caxpy(rv, a[t], -ar, -ai);
-    s_vec[t] = caxpy(sv, w[t], -ar, -ai);
-    p[t] = pn;
-    xout[t] = xn;
-    r[t] = rn;
+    if (!step2) return;
+    const T2 pn = caxpy(E.s, E.p, br, bi);
+    const T2 xn = caxpy(E.x, pn, ar, ai);
+    const T2 rn = caxpy(E.r, E.a, -ar, -ai);
+    st_nt(s_vec + t, caxpy(E.s, E.w, -ar, -ai));
+    st_nt(p + t, pn);
+    st_nt(xout + t, xn);
+    st_nt(r + t, rn);
     const T ur = rn.x - smv.x, ui = rn.y - smv.y;
     v[0] += double(smv.x) * double(ur) + double(smv.y) * double(ui);
     v[1] += double(ur) * double(ur) + double(ui) * double(ui);
+  };
+  for (long long tile = tw.first; run && tile < tw.last; tile += 2 * tw.step) {
+    const long long t0 = tile * kBlock + threadIdx.x;
+    const long long t1 = (tile + tw.step) * kBlock + threadIdx.x;
+    const bool ok0 = t0 < N;
+    const bool ok1 = tile + tw.step < tw.last && t1 < N;
+    Elem e0, e1;
+    if (ok0) load(t0, e0);
+    if (ok1) load(t1, e1);
+    if (ok0) update(t0, e0);
+    if (ok1) update(t1, e1);
   }
   block_partials<SP, 3>(v, partials);
 }
@@ -1431,8 +1497,9 @@ struct mtcocr_handle {
       }
   }
   mtcocr_handle() {
-    // row blocks for 4..16 systems, the row pack for 32 and 64 (set_caps clamps layout 2 there)
-    set_caps(8192, 2048, 8192, 2);
+    // row blocks for 4..16 systems, the row pack for 32 and 64 (set_caps clamps layout 2 there);
+    // streaming kernels at about one tile per workgroup (the grid is clamped to the tiles)
+    set_caps(8192, 65536, 2048, 2);
   }
 
   int64_t device_bytes() const {
@@ -2483,6 +2550,61 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
   return 0;
 }
 
+int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_ms,
+                        double* bytes_per_launch) {
+  if (!h || reps < 1 || !avg_ms || !bytes_per_launch) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || h->nsys < 1) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  const int sp = h->sp;
+  int rc = alloc_work(h, precision, sp);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, sp, 1e-8);
+  for (auto& q : hs) q.mode = MRS_BLEND;  // the common step: the smoothing blend and the update
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  const long long nv = (long long)h->nb * sp;
+  const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
+  float ms = 0.f;
+  int streams = 0;
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    auto run = [&](auto tag) -> int {
+      using T = decltype(tag);
+      using T2 = typename Vec2<T>::type;
+      Engine<T, SP> E(h);
+      for (T2* v : {E.x, E.x2, E.p, E.r, E.a, E.s, E.w, E.ym, E.sm}) k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+      T2* ym = E.ym_or_null();
+      streams = ym ? 14 : 12;
+      auto launch = [&](int i) {
+        k_xr_fused<T, SP><<<E.grid_e, kBlock, 0, st>>>(i & 1 ? E.x2 : E.x, i & 1 ? E.x : E.x2,
+                                                       E.p, E.r, E.a, E.s, E.w, ym, E.sm, E.sc,
+                                                       E.nrun, E.part, E.nb);
+      };
+      launch(1);
+      HIPC(hipEventRecord(h->ev0, st));
+      for (int i = 0; i < reps; ++i) launch(i);
+      HIPC(hipEventRecord(h->ev1, st));
+      return 0;
+    };
+    const int r = (precision == MTCOCR_PREC_FP64) ? run(double()) : run(float());
+    if (r) return r;
+    HIPC(hipGetLastError());
+    HIPC(hipEventSynchronize(h->ev1));
+    HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    return 0;
+  });
+  if (rc) return rc;
+  *avg_ms = double(ms) / reps;
+  // algorithmic bytes: each stream read or written once per system and edge
+  const double ev = (precision == MTCOCR_PREC_FP64) ? 16.0 : 8.0;
+  *bytes_per_launch = double(streams) * ev * double(h->nb) * double(h->nsys);
+  return 0;
+}
+
 int mtcocr_bench_copy(mtcocr_handle* h, int64_t nbytes, int reps, double* avg_ms) {
   if (!h || nbytes < 16 || nbytes % 16 || reps < 1 || !avg_ms) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
@@ -2568,7 +2690,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
   int one = 1;
   HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384};
+  // large grids: the streaming kernels reach their best bandwidth with about one 256-element
+  // tile per workgroup (the resident workgroups then sweep neighbouring tiles together)
+  static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
   const int ki = mtcocr_handle::sp_index(sp), q = precision;
   int* caps[3] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki]};
   int best[3] = {*caps[0], *caps[1], *caps[2]};

@@ -107,3 +107,17 @@ def test_torch_wrapper(problem):
     ref = problem.gradient(st) * sigma
     # the wrapper evaluates at exp(log(sigma)), one rounding away from sigma
     np.testing.assert_allclose(ls.grad.numpy(), ref, rtol=1e-8, atol=1e-9 * np.abs(ref).max())
+
+
+def test_directional_derivative(problem):
+    """A random +-1 direction in log(sigma) over the earth cells: grad . d against the central
+    difference of J along d."""
+    from mtcalc_with_dfp_cocr_amd.adjoint import _fd_direction
+
+    sigma = problem.m.sigma.copy()
+    grad = problem.gradient(problem.evaluate(sigma)) * sigma
+    xc, yc, zc = problem.m.cell_centres()
+    earth = zc < 0
+    d = np.where(earth, np.random.default_rng(9).choice([-1.0, 1.0], size=len(sigma)), 0.0)
+    fd = _fd_direction(problem, sigma, d, 1e-5)
+    assert grad @ d == pytest.approx(fd, rel=1e-6)
