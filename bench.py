@@ -87,13 +87,24 @@ def batched_only(m, freqs, args, label):
         S.set_mesh_order(ma, args.order_block)
     S.set_matrix_ks(p, kd, kg, sd, sg)
     S.set_gradient(ma)
-    S.set_rhs(prs, om)
-    tune = S.autotune("fp64", args.tune_reps)
+    # batches of at most max_batch systems, neighbouring frequencies together (they converge at
+    # similar rates), one after the other on the same handle
+    w = args.max_batch if args.max_batch > 0 else len(cases)
+    groups = [slice(i, min(i + w, len(cases))) for i in range(0, len(cases), w)]
+    tune = {}
+    for g in groups:
+        S.set_rhs(prs[g], om[g])
+        tune[f"{g.stop - g.start} systems"] = S.autotune("fp64", args.tune_reps)
     times = []
     st = None
     for k in range(args.warmup + args.steps):
-        S.set_rhs(prs, om)
-        dt, rc, st = timed(S, "fp64", args)
+        dt, st, rc = 0.0, [], 0
+        for g in groups:
+            S.set_rhs(prs[g], om[g])
+            d, r, q = timed(S, "fp64", args)
+            dt += d
+            rc = rc or r
+            st += q
         if k >= args.warmup:
             times.append(dt)
         log(f"{label}: {'warmup' if k < args.warmup else 'step'} {dt:.3f}s rc={rc}")
@@ -280,6 +291,8 @@ def main():
                     help="launches per candidate shape in the autotuner")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the headline (default); 3: four models x 32 systems; 5: 256x256x128")
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="configs 3 and 5: solve in batches of at most this many systems (0: all)")
     ap.add_argument("--adjoint-cells", type=int, default=4,
                     help="cells of the config-4 adjoint vs finite-difference check (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",

@@ -1884,6 +1884,9 @@ struct Fp64Run {
 template <class F>
 int with_sp(int sp, F&& f);
 
+// Widest batch the FP64 solver iterates at once; wider batches run as consecutive groups.
+constexpr int kGroupWidth = 16;
+
 // Runs the FP64 iterations of a batch of SP slots (slot j holds the caller's system orig[j]).
 // When the running systems fit into half the width, the finished ones are written to x64,
 // the running ones are repacked into a batch of the next power of two and the iterations
@@ -1993,22 +1996,9 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
   HIPC(hipMemcpyAsync(E.x, h->x64.p, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
   HIPC(hipMemcpyAsync(E.b, h->b64.p, (size_t)h->nb * SP * sizeof(double2), hipMemcpyDeviceToDevice, st));
   HIPC(hipStreamSynchronize(st));
-  // timed span: DFP setup (diagonals) + COCR loop, like the reference's kit interval
-  HIPC(hipEventRecord(h->ev0, st));
-  E.setup_diagonals(st);
-  E.prologue(st, maxiter);
-  HIPC(hipGetLastError());
-  Fp64Run R;
-  R.sp0 = SP;
-  R.final_sc.assign(SP, SysScalars());
-  R.done.assign(SP, 0);
-  std::vector<int> orig(SP, -1);
-  for (int s = 0; s < h->nsys; ++s) orig[s] = s;
-  rc = fp64_phase<SP>(h, maxiter, orig, om0, R);
-  if (rc) return rc;
-  {
-    // keep the wider set of work vectors in place for the next solve (their contents are
-    // spent: the answers are in x64)
+  // keep the wider set of work vectors as the main one (compaction swaps in narrower ones;
+  // their contents are spent once the answers are in x64)
+  auto keep_wider = [&]() {
     DevBuf* mains[] = {&h->vx, &h->vx2, &h->vr, &h->vp, &h->vs, &h->vz, &h->va, &h->vw,
                        &h->vym, &h->vsm};
     DevBuf* spares[] = {&h->cvx, &h->cvx2, &h->cvr, &h->cvp, &h->cvs, &h->cvz, &h->cva,
@@ -2019,6 +2009,70 @@ int solve_fp64(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats,
         std::swap(mains[i]->bytes, spares[i]->bytes);
         drop_graph(h);
       }
+  };
+  Fp64Run R;
+  R.sp0 = SP;
+  R.final_sc.assign(SP, SysScalars());
+  R.done.assign(SP, 0);
+  // timed span: DFP setup (diagonals) + COCR loop, like the reference's kit interval
+  HIPC(hipEventRecord(h->ev0, st));
+  if constexpr (SP <= kGroupWidth) {
+    E.setup_diagonals(st);
+    E.prologue(st, maxiter);
+    HIPC(hipGetLastError());
+    std::vector<int> orig(SP, -1);
+    for (int s = 0; s < h->nsys; ++s) orig[s] = s;
+    rc = fp64_phase<SP>(h, maxiter, orig, om0, R);
+    if (rc) return rc;
+    keep_wider();
+  } else {
+    // A wide batch runs as consecutive groups of kGroupWidth neighbouring systems: each group's
+    // x and b are repacked out of the caller's [row][system] arrays, the group runs (and
+    // narrows) on its own, and its answers are scattered back to x64. On config-3 models
+    // (32 systems) this was 8% faster than one batch of 32.
+    constexpr int GW = kGroupWidth;
+    if ((rc = ensure(h, h->cmap, sizeof(int) * 128))) return rc;
+    for (int g0 = 0; g0 < h->nsys; g0 += GW) {
+      const int cnt = std::min(GW, h->nsys - g0);
+      std::vector<int> map(GW, -1), orig(GW, -1);
+      std::vector<double> om(GW, 0.0);
+      std::vector<SysScalars> sc(GW);
+      for (int k = 0; k < GW; ++k) {
+        std::memset(&sc[k], 0, sizeof(SysScalars));
+        sc[k].status = sc[k].outer_status = ST_INACTIVE;
+      }
+      int active = 0;
+      for (int k = 0; k < cnt; ++k) {
+        map[k] = g0 + k;
+        om[k] = om0[g0 + k];
+        sc[k] = hs[g0 + k];
+        if (hs[g0 + k].status == ST_RUNNING) {
+          orig[k] = g0 + k;
+          ++active;
+        }
+      }
+      if (active == 0) continue;
+      Engine<double, GW> G(h);  // after keep_wider: the main vectors hold GW systems or more
+      const long long nv = (long long)h->nb * GW;
+      const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
+      HIPC(hipMemcpyAsync(h->cmap.p, map.data(), sizeof(int) * GW, hipMemcpyHostToDevice, st));
+      k_repack<double2><<<gv, 256, 0, st>>>(h->x64.as<double2>(), SP, G.x, GW,
+                                            h->cmap.as<int>(), h->nb);
+      k_repack<double2><<<gv, 256, 0, st>>>(h->b64.as<double2>(), SP, G.b, GW,
+                                            h->cmap.as<int>(), h->nb);
+      HIPC(hipMemcpyAsync(h->scal.p, sc.data(), sizeof(SysScalars) * GW, hipMemcpyHostToDevice,
+                          st));
+      HIPC(hipMemcpyAsync(h->omega.p, om.data(), sizeof(double) * GW, hipMemcpyHostToDevice, st));
+      HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+      drop_graph(h);  // a captured graph holds the previous width
+      G.setup_diagonals(st);
+      G.prologue(st, maxiter);
+      HIPC(hipGetLastError());
+      HIPC(hipStreamSynchronize(st));  // the host vectors above stay alive until here
+      rc = fp64_phase<GW>(h, maxiter, orig, om, R);
+      if (rc) return rc;
+      keep_wider();
+    }
   }
   HIPC(hipEventRecord(h->ev1, st));
   HIPC(hipEventSynchronize(h->ev1));

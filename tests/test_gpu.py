@@ -171,6 +171,40 @@ def test_batched_ks_matches_single(O, M, c1_model, c1_mesh):
         assert np.linalg.norm(b - A @ M.to_complex(xm[k])) / np.linalg.norm(b) < 1e-8
 
 
+def test_wide_batch_runs_in_groups(O, M, c1_model, c1_mesh):
+    """20 systems (batch width 32): the FP64 solver iterates them as a group of 16 and a group
+    of 4. The first group's answers are bit-identical to a batch of those 16 alone; every
+    system converges, the last ones agree with the oracle; the mixed solver converges too."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("wide batch: 20 systems in groups of 16")
+    freqs = M.frequencies(1e-2, 1e2, 10)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    with Solver(0) as B:
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        xw, sw = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        x16, s16 = B.solve_batch(om[:16], prs[:16], eps=1e-8, precision="fp64")
+        xm, sm = B.solve_batch(om, prs, eps=1e-8, precision="mixed")
+    assert all(q["req"] == 0 for q in sw) and all(q["req"] == 0 for q in sm)
+    assert np.array_equal(xw[:16], x16)
+    assert [q["iter"] for q in sw[:16]] == [q["iter"] for q in s16]
+    for k in (16, 19):
+        f, a = cases[k]
+        xo, so, _ = O.pcocr(M.assemble(c1_model, f, a), c1_mesh, eps=1e-8)
+        say(f"case {f:.3g} Hz alpha={a}: group {sw[k]['iter']} it, oracle {so['iter']} it")
+        assert abs(sw[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(xw[k], xo) <= 1e-5
+        s = M.assemble(c1_model, f, a)
+        res = M.to_complex(s.pr) - M.sparse_matrix(s) @ M.to_complex(xm[k])
+        assert np.linalg.norm(res) / np.linalg.norm(M.to_complex(s.pr)) < 1e-8
+
+
 def test_special_cases(S, M, c1_system, c1_mesh):
     say("special cases")
     x, st = S.solve(np.zeros_like(c1_system.pr), precision="fp64")
