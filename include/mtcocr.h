@@ -181,7 +181,9 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),
  * node and edge kernels at caps 512..131072, `reps` launches each, in `precision`, and keeps the
  * fastest choice of each family on the handle for this precision and batch width (padded system
- * count). Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the
+ * count; FP64 batches wider than 16 systems run their iterations as groups of 16 and are tuned
+ * at width 16, which is then what the outputs report; mtcocr_get_launch reports the width of
+ * the batch itself, which the kernel-level hooks use). Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the
  * chosen shape. Writes only work buffers: resident right-hand sides and solutions stay. */
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms);

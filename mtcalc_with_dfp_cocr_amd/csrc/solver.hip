@@ -2735,12 +2735,14 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
   if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
   HIPC(hipSetDevice(h->device));
-  const int sp = h->sp;
+  // FP64 batches wider than kGroupWidth iterate in groups of kGroupWidth: tune that width
+  const int sp =
+      (precision == MTCOCR_PREC_FP64 && h->sp > kGroupWidth) ? kGroupWidth : h->sp;
   int rc = alloc_work(h, precision, sp);
   if (rc) return rc;
   hipStream_t st = h->stream;
   std::vector<SysScalars> hs;
-  set_eps_status(h, hs, h->nsys, sp, 1e-8);
+  set_eps_status(h, hs, std::min(h->nsys, sp), sp, 1e-8);
   HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
   int one = 1;
   HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
