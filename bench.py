@@ -437,7 +437,7 @@ def main():
         log(f"adjoint (config 4): {adj} ({time.perf_counter() - t0:.1f}s)")
 
     # ---- CPU baseline -------------------------------------------------------------------
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu and fp64:
         threads = max(1, min(os.cpu_count() or 1, 16) - 1)
         k = min(range(len(fp64)), key=lambda i: abs(np.log10(fp64[i]["f"])))  # nearest 1 Hz
@@ -451,6 +451,15 @@ def main():
                           f"{per_iter * 1e3:.1f} ms/iter x {fp64[k]['iter']} iterations "
                           f"(the FP64 GPU count); {spent:.1f}s of CPU work")}
         log(f"cpu baseline: {cpu}")
+        # the same on every core the box gives this process (the reference's own thread cap is
+        # one below: ControlOMP.cpp:46)
+        allc = min(os.cpu_count() or 1, 16)
+        if allc > threads:
+            est2, per2, setup2, spent2 = cpu_baseline(M, m, ma, cases[k], fp64[k]["iter"],
+                                                      args.cpu_iters, allc)
+            cpu_all = {"value": est2, "cores": allc, "per_iter_ms": per2 * 1e3,
+                       "cpu_seconds": spent2}
+            log(f"cpu baseline, all cores: {cpu_all}")
 
     spb = 1 << (nsys - 1).bit_length()
     traffic = None if args.no_traffic else measure_traffic(args, spb)
@@ -492,6 +501,7 @@ def main():
         "spmv_fp64_batched": spmv_line,
         "update_fp64_batched": upd_line,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "fp64_gpu_per_system_s": fp64_per_sys,
         "speedup_vs_fp64_gpu": (fp64_per_sys / value) if fp64_per_sys else None,
         "batched_ok": batched_ok,

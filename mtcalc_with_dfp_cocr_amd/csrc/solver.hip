@@ -2126,7 +2126,10 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
     int* nrun_outer = h->nrun2.as<int>();
     HIPC(hipMemcpyAsync(nrun_outer, &one, sizeof(int), hipMemcpyHostToDevice, st));
     HIPC(hipStreamSynchronize(st));
-    const double inner_floor = 1e-5;
+    // Each FP32 pass aims at most two orders of magnitude below the residual it starts from.
+    // In the air the FP32 vectors stall well above eps, so the FP64 finish does the rest: on
+    // the config-2 batch a 1e-2 floor took 0.22 s per system, 1e-3 0.25 s and 1e-5 0.30 s.
+    const double inner_floor = 1e-2;
     const int max_outer = 40;
     HIPC(hipEventRecord(h->ev0, st));
     E.setup_diagonals(st);
