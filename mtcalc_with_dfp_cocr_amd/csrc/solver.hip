@@ -2730,7 +2730,7 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 //                   blocks k_spmv_blk for 4..16 systems) x cap
 //                   (the cap also sets the partial count of its reductions)
 //   node pass       k_pc_node cap
-//   edge streams    k_pc_edge_iter cap, which stands for k_xr_fused as well
+//   edge streams    k_pc_edge_iter + k_xr_fused (they share the cap), timed together
 // Only work buffers are written; the resident right-hand sides and solutions are untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms) {
@@ -2746,6 +2746,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   hipStream_t st = h->stream;
   std::vector<SysScalars> hs;
   set_eps_status(h, hs, std::min(h->nsys, sp), sp, 1e-8);
+  for (auto& q : hs) q.mode = MRS_BLEND;  // the update pass's common step
   HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
   int one = 1;
   HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
@@ -2767,7 +2768,8 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         const long long nv = (long long)E.nb * SP, nm = (long long)std::max(E.m, 1) * SP;
         const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
         const int gm = (int)std::min<long long>(2048, (nm + 255) / 256);
-        for (T2* v : {E.s, E.z, E.a, E.p, E.w}) k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+        for (T2* v : {E.s, E.z, E.a, E.p, E.w, E.x, E.x2, E.r, E.ym, E.sm})
+          k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
         k_fill<T2><<<gm, 256, 0, st>>>(E.help, nm);
         E.setup_diagonals(st);
         HIPC(hipGetLastError());
@@ -2834,6 +2836,11 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 int nr;
                 const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
                 s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
+                k_xr_fused<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.x, E.x2, E.p, E.r, E.a, E.s,
+                                                               E.w, E.ym_or_null(), E.sm, E.sc,
+                                                               E.nrun, E.part, E.nb);
+                pp = E.template partials<SP * 3>(st, E.grid_e, nr);
+                s_reduce_probe<SP * 3><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               }
             };
             launch();
