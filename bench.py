@@ -287,7 +287,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--order-block", type=int, default=16,
                     help="pencil width of the device's locality order (0: the file order)")
-    ap.add_argument("--tune-reps", type=int, default=5,
+    ap.add_argument("--tune-reps", type=int, default=10,
                     help="launches per candidate shape in the autotuner")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the headline (default); 3: four models x 32 systems; 5: 256x256x128")

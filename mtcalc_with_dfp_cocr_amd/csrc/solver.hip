@@ -856,7 +856,34 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
     if (t >= N) continue;
     const int i = int(t / SP);
     T hr = 0, hi = 0;
-    for (int k = 0; k < G.wg; ++k) {
+    // the first kGU entries (a node of a hexahedral mesh has at most 6 edges) with every load
+    // issued before the arithmetic, the rest (other meshes) one by one; the sum keeps the order
+    constexpr int kGU = 6;
+    int ec[kGU];
+    T gv[kGU];
+    T2 zv[kGU], av[kGU];
+#pragma unroll
+    for (int k = 0; k < kGU; ++k) {
+      ec[k] = k < G.wg ? G.gcol[(size_t)k * G.m + i] : 0;
+      gv[k] = k < G.wg ? G.gval[(size_t)k * G.m + i] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kGU; ++k) {
+      if (k < G.wg) {
+        zv[k] = z[(size_t)ec[k] * SP + s];
+        if (USE_BETA) av[k] = a[(size_t)ec[k] * SP + s];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGU; ++k) {
+      if (k < G.wg) {
+        T2 vv = zv[k];
+        if (USE_BETA) vv = caxpy(vv, av[k], br, bi);
+        hr += gv[k] * vv.x;
+        hi += gv[k] * vv.y;
+      }
+    }
+    for (int k = kGU; k < G.wg; ++k) {
       const int e = G.gcol[(size_t)k * G.m + i];
       const T g = G.gval[(size_t)k * G.m + i];
       T2 vv = z[(size_t)e * SP + s];
@@ -869,6 +896,40 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
     o.x = hr * d.x - hi * d.y;
     o.y = hi * d.x + hr * d.y;
     help[t] = o;
+  }
+}
+
+// y += (G^T help)_e for one system: the first two entries (an edge has two end nodes) with
+// their loads issued together, any further ones (other meshes) one by one, in entry order
+template <class T, int SP>
+__device__ inline void gt_apply(const GView<T>& G, const typename Vec2<T>::type* __restrict__ help,
+                                int e, int s, T& yr, T& yi) {
+  using T2 = typename Vec2<T>::type;
+  constexpr int kGU = 2;
+  int nc[kGU];
+  T gv[kGU];
+  T2 hv[kGU];
+#pragma unroll
+  for (int k = 0; k < kGU; ++k) {
+    nc[k] = k < G.wgt ? G.gtcol[(size_t)k * G.n + e] : 0;
+    gv[k] = k < G.wgt ? G.gtval[(size_t)k * G.n + e] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kGU; ++k)
+    if (k < G.wgt) hv[k] = help[(size_t)nc[k] * SP + s];
+#pragma unroll
+  for (int k = 0; k < kGU; ++k) {
+    if (k < G.wgt) {
+      yr += gv[k] * hv[k].x;
+      yi += gv[k] * hv[k].y;
+    }
+  }
+  for (int k = kGU; k < G.wgt; ++k) {
+    const int node = G.gtcol[(size_t)k * G.n + e];
+    const T g = G.gtval[(size_t)k * G.n + e];
+    const T2 h = help[(size_t)node * SP + s];
+    yr += g * h.x;
+    yi += g * h.y;
   }
 }
 
@@ -892,13 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
     const T2 xv = v[t], d = diag_inv<T>(da, e, s);
     T yr = xv.x * d.x - xv.y * d.y;
     T yi = xv.x * d.y + xv.y * d.x;
-    for (int k = 0; k < G.wgt; ++k) {
-      const int node = G.gtcol[(size_t)k * G.n + e];
-      const T g = G.gtval[(size_t)k * G.n + e];
-      const T2 h = help[(size_t)node * SP + s];
-      yr += g * h.x;
-      yi += g * h.y;
-    }
+    gt_apply<T, SP>(G, help, e, s, yr, yi);
     T2 o;
     o.x = yr;
     o.y = yi;
@@ -930,13 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     const T2 d = diag_inv<T>(da, e, s);
     T yr = an.x * d.x - an.y * d.y;
     T yi = an.x * d.y + an.y * d.x;
-    for (int k = 0; k < G.wgt; ++k) {
-      const int node = G.gtcol[(size_t)k * G.n + e];
-      const T g = G.gtval[(size_t)k * G.n + e];
-      const T2 h = help[(size_t)node * SP + s];
-      yr += g * h.x;
-      yi += g * h.y;
-    }
+    gt_apply<T, SP>(G, help, e, s, yr, yi);
     T2 o;
     o.x = yr;
     o.y = yi;
