@@ -169,9 +169,10 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
  * that the sliced ELL runs. set_launch applies to both precisions and every batch width;
  * get_launch reports those of `precision` at the current batch width, the layout as the one
  * that runs. Caps change only the reduction grouping of the dot products and the layout only
- * the summation order inside a row (results are deterministic for a fixed shape). Defaults:
- * caps 8192 / 65536 / 2048, sliced ELL for 1..2 systems, row blocks for 3..16, row pack for
- * 17..64. */
+ * the summation order inside a row (results are deterministic for a fixed shape). cap_vec
+ * also sets the fused update pass's cap, which the autotuner tunes apart. Defaults: caps
+ * 8192 / 65536 / 2048 (update pass 131072), sliced ELL for 1..2 systems, row blocks for 3..16,
+ * row pack for 17..64. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
@@ -179,12 +180,13 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 
 /* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),
- * node and edge kernels at caps 512..131072, `reps` launches each, in `precision`, and keeps the
- * fastest choice of each family on the handle for this precision and batch width (padded system
- * count; FP64 batches wider than 16 systems run their iterations as groups of 16 and are tuned
- * at width 16, which is then what the outputs report; mtcocr_get_launch reports the width of
- * the batch itself, which the kernel-level hooks use). Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the
- * chosen shape. Writes only work buffers: resident right-hand sides and solutions stay. */
+ * node, edge and update-pass kernels at caps 512..131072, `reps` launches each, in
+ * `precision`, and keeps the fastest choice of each family on the handle for this precision
+ * and batch width (padded system count). FP64 batches wider than 16 systems run their
+ * iterations as groups of 16: they are tuned at width 16, which the outputs then report, while
+ * mtcocr_get_launch reports the width of the batch itself (used by the kernel-level hooks).
+ * Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the chosen
+ * shape. Writes only work buffers: resident right-hand sides and solutions stay. */
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms);
 

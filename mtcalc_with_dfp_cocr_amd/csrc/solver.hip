@@ -1527,6 +1527,8 @@ struct mtcocr_handle {
   // launch shapes (workgroup caps of the grid-stride kernels) per vector precision (FP64,
   // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
   int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
+  // the fused update pass's own cap: set with cap_vec by set_launch, tuned apart by autotune
+  int cap_upd[2][7];
   // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg,
   // batch widths 4..64), 2 halo-staged row blocks (k_spmv_blk, batch widths 4..16; where it
   // does not apply, the row pack runs)
@@ -1540,7 +1542,7 @@ struct mtcocr_handle {
     for (int p = 0; p < 2; ++p)
       for (int k = 0; k < 7; ++k) {
         if (cs) cap_spmv[p][k] = cs;
-        if (cv) cap_vec[p][k] = cv;
+        if (cv) cap_vec[p][k] = cap_upd[p][k] = cv;
         if (cn) cap_node[p][k] = cn;
         if (layout >= 0) spmv_layout[p][k] = (k < 2) ? 0 : (k > 4 && layout > 1) ? 1 : layout;
       }
@@ -1549,6 +1551,8 @@ struct mtcocr_handle {
     // row blocks for 4..16 systems, the row pack for 32 and 64 (set_caps clamps layout 2 there);
     // streaming kernels at about one tile per workgroup (the grid is clamped to the tiles)
     set_caps(8192, 65536, 2048, 2);
+    for (auto& row : cap_upd)
+      for (int& c : row) c = kMaxBlocks;  // the update pass: one tile per workgroup
   }
 
   int64_t device_bytes() const {
@@ -1730,7 +1734,7 @@ struct Engine {
   int* nrun;
   double *part, *part2;
   const double* omega;
-  int nb, m, grid_s, grid_e, grid_n, layout;
+  int nb, m, grid_s, grid_e, grid_n, grid_u, layout;
 
   explicit Engine(mtcocr_handle* hh) : h(hh) {
     A = ell_view<double>(h);
@@ -1761,6 +1765,7 @@ struct Engine {
     const int k = mtcocr_handle::sp_index(SP), q = std::is_same<T, float>::value ? 1 : 0;
     grid_s = grid_for((long long)nb * SP, h->cap_spmv[q][k]);
     grid_e = grid_for((long long)nb * SP, h->cap_vec[q][k]);
+    grid_u = grid_for((long long)nb * SP, h->cap_upd[q][k]);
     grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
     layout = h->spmv_layout[q][k];
   }
@@ -1822,9 +1827,9 @@ struct Engine {
     int nr;
     const double* pp = partials<SP * 2>(st, grid_e, nr);
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    k_xr_fused<T, SP><<<grid_e, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w,
+    k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w,
                                                  ym_or_null(), sm, sc, nrun, part, nb);
-    pp = partials<SP * 3>(st, grid_e, nr);
+    pp = partials<SP * 3>(st, grid_u, nr);
     s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     pp = partials<SP * 2>(st, grid_s, nr);
@@ -2686,7 +2691,7 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
       T2* ym = E.ym_or_null();
       streams = ym ? 14 : 12;
       auto launch = [&](int i) {
-        k_xr_fused<T, SP><<<E.grid_e, kBlock, 0, st>>>(i & 1 ? E.x2 : E.x, i & 1 ? E.x : E.x2,
+        k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(i & 1 ? E.x2 : E.x, i & 1 ? E.x : E.x2,
                                                        E.p, E.r, E.a, E.s, E.w, ym, E.sm, E.sc,
                                                        E.nrun, E.part, E.nb);
       };
@@ -2779,7 +2784,8 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 //                   blocks k_spmv_blk for 4..16 systems) x cap
 //                   (the cap also sets the partial count of its reductions)
 //   node pass       k_pc_node cap
-//   edge streams    k_pc_edge_iter + k_xr_fused (they share the cap), timed together
+//   edge streams    k_pc_edge_iter cap (reported as cap_vec)
+//   update pass     k_xr_fused, tuned apart (set_launch's cap_vec sets both)
 // Only work buffers are written; the resident right-hand sides and solutions are untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms) {
@@ -2803,10 +2809,11 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   // tile per workgroup (the resident workgroups then sweep neighbouring tiles together)
   static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
   const int ki = mtcocr_handle::sp_index(sp), q = precision;
-  int* caps[3] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki]};
-  int best[3] = {*caps[0], *caps[1], *caps[2]};
+  int* caps[4] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki],
+                  &h->cap_upd[q][ki]};
+  int best[4] = {*caps[0], *caps[1], *caps[2], *caps[3]};
   int best_layout = h->spmv_layout[q][ki];
-  double best_ms[3] = {1e30, 1e30, 1e30};
+  double best_ms[4] = {1e30, 1e30, 1e30, 1e30};
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
     auto run = [&](auto tag) -> int {
@@ -2823,7 +2830,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         E.setup_diagonals(st);
         HIPC(hipGetLastError());
       }
-      for (int fam = 0; fam < 3; ++fam) {
+      for (int fam = 0; fam < 4; ++fam) {
         const int nlayouts =
             (fam != 0 || SP < 4 || !rows_fit(h, SP, int(sizeof(T2)))) ? 1
             : (SP <= 16 && blk_lds_bytes<T>(h) > 0)                  ? 3
@@ -2848,9 +2855,12 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
           } else if (fam == 1) {
             HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per_cu, reinterpret_cast<const void*>(k_pc_node<T, SP, true>), kBlock, 0));
-          } else {
+          } else if (fam == 2) {
             HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per_cu, reinterpret_cast<const void*>(k_pc_edge_iter<T, SP>), kBlock, 0));
+          } else {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_xr_fused<T, SP>), kBlock, 0));
           }
           std::vector<int> cands(std::begin(kCands), std::end(kCands));
           const int resident = round_up(std::max(1, per_cu * h->n_cu), kXcd);
@@ -2863,7 +2873,8 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
             *caps[fam] = cap;
             if (fam == 0) h->spmv_layout[q][ki] = layout;
             Engine<T, SP> E(h);
-            const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : E.grid_e;
+            const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : fam == 2 ? E.grid_e
+                                                                                   : E.grid_u;
             if (grid == last_grid) continue;  // the cap no longer limits this grid
             last_grid = grid;
             // each timed unit is the kernel plus the one-block reduction of its partials
@@ -2878,17 +2889,19 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
               } else if (fam == 1) {
                 k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
                                                                     E.sc, E.nrun);
-              } else {
+              } else if (fam == 2) {
                 k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.w,
                                                                    E.da, E.help, E.sc, E.nrun,
                                                                    E.part);
                 int nr;
                 const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
                 s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
-                k_xr_fused<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.x, E.x2, E.p, E.r, E.a, E.s,
+              } else {
+                k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.x2, E.p, E.r, E.a, E.s,
                                                                E.w, E.ym_or_null(), E.sm, E.sc,
                                                                E.nrun, E.part, E.nb);
-                pp = E.template partials<SP * 3>(st, E.grid_e, nr);
+                int nr;
+                const double* pp = E.template partials<SP * 3>(st, E.grid_u, nr);
                 s_reduce_probe<SP * 3><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               }
             };
