@@ -511,3 +511,27 @@ def test_impedance_parity_with_oracle(O, M, c1_model, c1_mesh):
     for i, j in off:
         np.testing.assert_allclose(rg[:, i, j], ro[:, i, j], rtol=2e-6)
         np.testing.assert_allclose(pg[:, i, j], po[:, i, j], atol=1e-4)
+
+
+def test_launcher_end_to_end(M, c1_model, tmp_path):
+    """CalcStarter-equivalent run on config 1 at two frequencies: the working directories, one
+    batched COCR_FP launch, the impedance table; the same Z (to 1e-6) as one solver process
+    per directory, the reference's way."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd import launcher as L
+
+    say("launcher: config 1, 2 frequencies")
+    freqs = [0.3, 3.0]
+    rec = np.array([[x, y, 0.0] for x in (-100.0, 0.0, 100.0) for y in (-100.0, 100.0)])
+    za = tmp_path / "batched"
+    zb = tmp_path / "per_directory"
+    assert L.run(L.prepare(za, c1_model, freqs), batch=True, timeout=300) == 0
+    Za = L.collect(za, c1_model, freqs, rec)
+    assert L.run(L.prepare(zb, c1_model, freqs), batch=False, timeout=300) == 0
+    Zb = L.collect(zb, c1_model, freqs, rec)
+    lines = (za / "Results" / "impedance.txt").read_text().splitlines()
+    assert len(lines) == len(freqs) * len(rec) and len(lines[0].split("\t")) == 15
+    for i, j in ((0, 1), (1, 0)):
+        d = np.abs(Za[:, :, i, j] - Zb[:, :, i, j]) / np.abs(Zb[:, :, i, j])
+        assert d.max() <= 1e-6
