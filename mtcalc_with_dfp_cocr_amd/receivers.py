@@ -1,6 +1,6 @@
 """Receiver fields and the impedance tensor in FP64.
 
-Restates what follows the solver in the reference for the quantities the solver is judged by:
+Restates what follows the solver in the reference, for the quantities its results are compared by:
   fields   E = -i*omega*(A + A0), H = curl(A + A0)/mu0 at the receivers
            (Code/OutputNu/give_out_vec_mt.cpp:101-272: E from the vector potential, H from its
            curl, the normal field of Harm1D added back; Harm1D writes the normal H as mu*alpha /
