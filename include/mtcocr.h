@@ -21,6 +21,8 @@
  *   mtcocr_set_launch, mtcocr_get_launch, mtcocr_autotune
  *                             no counterpart (launch shapes of the GPU kernels; the reference
  *                             is a CPU OpenMP solver with no tunable launch).
+ *   mtcocr_sigma_gradient     no counterpart (the conductivity gradient of an impedance misfit
+ *                             after the adjoint solves; the reference computes no gradient).
  *
  * Conventions: all pointers are host pointers to caller-owned memory; indices are 0-based int32
  * (the reference decrements its 1-based files before the call, HarmVect3d.cpp:203-226); complex
@@ -216,6 +218,20 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
  * zeros dropped), number of interior nodes m of G, device bytes held by the handle. */
 int mtcocr_matrix_info(mtcocr_handle* h, int64_t* nnz_complex, int64_t* nnz_real, int64_t* m,
                        int64_t* device_bytes);
+
+/* Conductivity gradient contraction of the adjoint, on the device. No counterpart in the
+ * reference (it computes no gradient); it completes dJ/dsigma after the adjoint solves:
+ *   grad[c] = sum_s Re( coef_s * u_s^T M_c v_s )
+ * over the cells c of a regular grid of nxn x nyn x nzn nodes (coordinates x, y, z; cells with
+ * x fastest), M_c the cell's 12x12 edge mass matrix without sigma (the same matrix as
+ * mtmodel_element_matrices' c: dA/dsigma_c = i*omega M_c), u_s and v_s complex interleaved
+ * edge vectors of system s in the mesh writer's edge order (host arrays of nsys x 2*n_edges),
+ * coef_s complex interleaved (2*nsys). For the impedance misfit, u_s = the adjoint solution,
+ * v_s = the total field (solution plus normal-field DOFs) and coef_s = -2i*omega_s. One thread
+ * per cell sums its systems in order (deterministic). Needs no matrix on the handle. */
+int mtcocr_sigma_gradient(mtcocr_handle* h, int nxn, int nyn, int nzn, const double* x,
+                          const double* y, const double* z, int nsys, const double* coef,
+                          const double* u, const double* v, double* grad);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,8 @@ SIGNATURES = {
     "mtcocr_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f64p, f64p]),
     "mtcocr_matrix_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mtcocr_sigma_gradient": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, f64p, f64p, f64p,
+                                        C.c_int, f64p, f64p, f64p, f64p]),
 }
 
 

@@ -183,6 +183,23 @@ class Solver:
                                    _f64(x), y), "mtcocr_apply")
         return y
 
+    def sigma_gradient(self, shape_nodes, x, y, z, coef, u, v) -> np.ndarray:
+        """grad[c] = sum_s Re(coef_s * u_s^T M_c v_s) per cell of the regular grid, on the device
+        (M_c: the cell's edge mass matrix; u, v: (nsys, n_edges) complex; coef: (nsys,))."""
+        nxn, nyn, nzn = (int(n) for n in shape_nodes)
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        nsys = u.shape[0]
+        if u.ndim != 2 or v.shape != u.shape or len(coef) != nsys:
+            raise ValueError("u, v and coef must have one row / entry per system")
+        ci = np.ascontiguousarray(coef, dtype=np.complex128).view(np.float64)
+        out = np.empty((nxn - 1) * (nyn - 1) * (nzn - 1))
+        check(self._L.mtcocr_sigma_gradient(self._h, nxn, nyn, nzn, _f64(x), _f64(y), _f64(z),
+                                            nsys, ci, u.view(np.float64).reshape(-1),
+                                            v.view(np.float64).reshape(-1), out),
+              "mtcocr_sigma_gradient")
+        return out
+
     def info(self) -> dict:
         a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         check(self._L.mtcocr_matrix_info(self._h, C.byref(a), C.byref(b), C.byref(c),

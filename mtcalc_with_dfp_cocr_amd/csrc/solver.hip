@@ -35,6 +35,7 @@
 
 #include "host_prep.h"
 #include "mtcocr.h"
+#include "mtmodel.h"
 
 namespace mtcocr {
 
@@ -255,6 +256,68 @@ __global__ void k_fill(T2* v, long long n) {
     z.x = 0;
     z.y = 0;
     v[k] = z;
+  }
+}
+
+// Conductivity gradient contraction of the adjoint: grad[c] = sum_s Re(coef_s * u_s^T M_c v_s)
+// for the cells of a regular grid (x fastest), one thread per cell. M_c is the lowest-order
+// edge mass matrix of the brick hx*hy*hz in the mesh writer's local edge order (edges 0-3 along
+// x, 4-7 along y, 8-11 along z); only edges of one direction couple. Its entries are formed as
+// the model layer's element integrals form them: for edges i, j of direction d, sitting at
+// signs s_i, s_j in the two other coordinates of the reference cube [-1,1]^3,
+//   M_ij = (1/(2 h_d))^2 * prod_k I_k * hx hy hz / 8,  I_d = 2,  I_k = 2 + (2/3) s_ik s_jk,
+// the integral of the product of the two basis functions (edge DOF = tangential field * h/2).
+// ce holds the 12 edges of every cell; u and v are [system][edge] complex.
+__global__ __launch_bounds__(256) void k_sigma_grad(
+    const int* __restrict__ ce, const double* __restrict__ xn, const double* __restrict__ yn,
+    const double* __restrict__ zn, int ncx, int ncy, long long ncells, int nsys,
+    const double* __restrict__ coef, const double2* __restrict__ u,
+    const double2* __restrict__ v, long long nb, double* __restrict__ grad) {
+#pragma clang fp contract(off)
+  // position signs of the local edges in the two coordinates other than their direction
+  constexpr signed char kSign[12][3] = {
+      {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1},  {-1, 0, -1}, {-1, 0, 1},
+      {1, 0, -1},  {1, 0, 1},  {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0}};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += stride) {
+    const int cx = int(c % ncx), cy = int((c / ncx) % ncy), cz = int(c / ((long long)ncx * ncy));
+    const double h[3] = {xn[cx + 1] - xn[cx], yn[cy + 1] - yn[cy], zn[cz + 1] - zn[cz]};
+    const double detj = h[0] * h[1] * h[2] / 8.0;
+    double m[3][4][4];
+    for (int d = 0; d < 3; ++d) {
+      const double base = 0.5 / h[d];
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+          double t = base * base;
+          for (int k = 0; k < 3; ++k)
+            t *= 2.0 + (2.0 / 3.0) * double(kSign[4 * d + i][k]) * double(kSign[4 * d + j][k]);
+          m[d][i][j] = t * detj;
+        }
+    }
+    int e[12];
+    for (int k = 0; k < 12; ++k) e[k] = ce[c * 12 + k];
+    double g = 0.0;
+    for (int s = 0; s < nsys; ++s) {
+      const double2* us = u + (size_t)s * nb;
+      const double2* vs = v + (size_t)s * nb;
+      double fr = 0.0, fi = 0.0;  // u^T M v, complex, unconjugated
+      for (int d = 0; d < 3; ++d) {
+        double2 vv[4];
+        for (int k = 0; k < 4; ++k) vv[k] = vs[e[4 * d + k]];
+        for (int i = 0; i < 4; ++i) {
+          const double2 uu = us[e[4 * d + i]];
+          double tr = 0.0, ti = 0.0;
+          for (int k = 0; k < 4; ++k) {
+            tr += m[d][i][k] * vv[k].x;
+            ti += m[d][i][k] * vv[k].y;
+          }
+          fr += uu.x * tr - uu.y * ti;
+          fi += uu.x * ti + uu.y * tr;
+        }
+      }
+      g += coef[2 * s] * fr - coef[2 * s + 1] * fi;
+    }
+    grad[c] = g;
   }
 }
 
@@ -1529,6 +1592,9 @@ struct mtcocr_handle {
   // the same vectors for a narrowed batch (swapped with the above by compaction)
   DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
   DevBuf cmap;  // slot maps of compaction / scatter
+  // the adjoint's gradient contraction (mtcocr_sigma_gradient): cell edges, node coordinates,
+  // coefficients, the two vector batches and the per-cell result
+  DevBuf sg_ce, sg_xyz, sg_coef, sg_u, sg_v, sg_out;
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
   int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
@@ -1569,7 +1635,7 @@ struct mtcocr_handle {
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2, &cvr, &cvp, &cvs, &cvz, &cva, &cvw, &cvym,
                            &cvsm, &cvx, &cvx2, &cmap,
-                           &vx, &vb};
+                           &vx, &vb, &sg_ce, &sg_xyz, &sg_coef, &sg_u, &sg_v, &sg_out};
     int64_t t = 0;
     for (auto* b : all) t += (int64_t)b->bytes;
     return t;
@@ -3021,6 +3087,54 @@ int mtcocr_matrix_info(mtcocr_handle* h, int64_t* nnz_complex, int64_t* nnz_real
   if (nnz_real) *nnz_real = h->nnz_r;
   if (m) *m = h->m;
   if (device_bytes) *device_bytes = h->device_bytes();
+  return 0;
+}
+
+int mtcocr_sigma_gradient(mtcocr_handle* h, int nxn, int nyn, int nzn, const double* x,
+                          const double* y, const double* z, int nsys, const double* coef,
+                          const double* u, const double* v, double* grad) {
+  if (!h || !x || !y || !z || !coef || !u || !v || !grad || nsys < 1) return MTCOCR_ERR_ARG;
+  if (nxn < 2 || nyn < 2 || nzn < 2) return MTCOCR_ERR_ARG;
+  const long long ncx = nxn - 1, ncy = nyn - 1, ncz = nzn - 1, ncells = ncx * ncy * ncz;
+  const long long nb = (long long)mtmodel_num_edges(nxn, nyn, nzn);
+  if (nb <= 0 || nb > 2147483000LL) return MTCOCR_ERR_ARG;
+  // the cells' edges in the mesh writer's local order (edges.dat rows, 1-based)
+  std::vector<int> ed((size_t)ncells * 25), ce((size_t)ncells * 12);
+  int64_t kt1 = 0;
+  if (mtmodel_topology(nxn, nyn, nzn, nullptr, ed.data(), nullptr, nullptr, &kt1) != 0)
+    return MTCOCR_ERR_ARG;
+  for (long long c = 0; c < ncells; ++c)
+    for (int k = 0; k < 12; ++k) ce[(size_t)c * 12 + k] = ed[(size_t)c * 25 + k] - 1;
+  std::vector<double> xyz;
+  xyz.reserve((size_t)nxn + nyn + nzn);
+  xyz.insert(xyz.end(), x, x + nxn);
+  xyz.insert(xyz.end(), y, y + nyn);
+  xyz.insert(xyz.end(), z, z + nzn);
+  HIPC(hipSetDevice(h->device));
+  hipStream_t st = h->stream;
+  const size_t vbytes = (size_t)nsys * nb * 2 * sizeof(double);
+  auto put = [&](DevBuf& d, const void* src, size_t bytes) -> int {
+    if (d.bytes < bytes || !d.p) HIPC(d.alloc(bytes));
+    if (src) HIPC(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, st));
+    return 0;
+  };
+  int rc;
+  if ((rc = put(h->sg_ce, ce.data(), ce.size() * sizeof(int))) ||
+      (rc = put(h->sg_xyz, xyz.data(), xyz.size() * sizeof(double))) ||
+      (rc = put(h->sg_coef, coef, (size_t)nsys * 2 * sizeof(double))) ||
+      (rc = put(h->sg_u, u, vbytes)) || (rc = put(h->sg_v, v, vbytes)) ||
+      (rc = put(h->sg_out, nullptr, (size_t)ncells * sizeof(double))))
+    return rc;
+  const double* xd = h->sg_xyz.as<double>();
+  const int grid = (int)std::min<long long>((ncells + 255) / 256, 65536);
+  k_sigma_grad<<<grid, 256, 0, st>>>(h->sg_ce.as<int>(), xd, xd + nxn, xd + nxn + nyn, int(ncx),
+                                     int(ncy), ncells, nsys, h->sg_coef.as<double>(),
+                                     h->sg_u.as<double2>(), h->sg_v.as<double2>(), nb,
+                                     h->sg_out.as<double>());
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(grad, h->sg_out.p, (size_t)ncells * sizeof(double), hipMemcpyDeviceToHost,
+                      st));
+  HIPC(hipStreamSynchronize(st));
   return 0;
 }
 

@@ -471,6 +471,34 @@ def test_adjoint_gradient_gpu(M, small_model):
     assert err <= 1e-6
 
 
+def test_sigma_gradient_contraction(M, small_model):
+    """The device contraction sum_s Re(coef_s u_s^T M_c v_s) against the host's per-cell mass
+    forms (the element mass matrices of the model layer), on random vectors; the host sums in
+    the same order without FMA, so the two agree to rounding."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.adjoint import cell_mass_forms
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("sigma gradient contraction")
+    m = small_model
+    rng = np.random.default_rng(5)
+    nsys, nb = 3, m.n_edges
+    u = rng.standard_normal((nsys, nb)) + 1j * rng.standard_normal((nsys, nb))
+    v = rng.standard_normal((nsys, nb)) + 1j * rng.standard_normal((nsys, nb))
+    coef = rng.standard_normal(nsys) + 1j * rng.standard_normal(nsys)
+    ref = np.zeros(m.n_cells)
+    for s in range(nsys):
+        ref += np.real(coef[s] * cell_mass_forms(m, u[s], v[s]))
+    with Solver(0) as H:  # needs no matrix
+        g = H.sigma_gradient(m.shape_nodes, m.x, m.y, m.z, coef, u, v)
+        g1 = H.sigma_gradient(m.shape_nodes, m.x, m.y, m.z, coef[:1], u[:1], v[:1])
+    ref1 = np.real(coef[0] * cell_mass_forms(m, u[0], v[0]))
+    say(f"max |gpu - host| {np.abs(g - ref).max():.3e} of {np.abs(ref).max():.3e}")
+    assert np.abs(g - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.abs(g1 - ref1).max() <= 1e-13 * np.abs(ref1).max()
+
+
 def test_impedance_parity_with_oracle(O, M, c1_model, c1_mesh):
     """Z (and rho_a, phase) at the 25 config-1 receivers from the GPU solutions of both
     polarizations against Z from the oracle's solutions: within 1e-6 relative."""
