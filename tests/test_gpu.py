@@ -459,8 +459,19 @@ def test_adjoint_gradient_gpu(M, small_model):
         prob = ImpedanceMisfit(m, freqs, rec, z_obs=zobs, solver=gpu)
         st2 = prob.evaluate(m.sigma)
         g = prob.gradient(st2)
+        # the torch.autograd wrapper over the same GPU problem: J forward, the adjoint backward
+        import torch
+
+        from mtcalc_with_dfp_cocr_amd.adjoint import torch_misfit
+
+        ls = torch.tensor(np.log(m.sigma), requires_grad=True)
+        J = torch_misfit(prob)(ls)
+        J.backward()
+        gt = ls.grad.numpy()
     finally:
         gpu.close()
+    assert float(J.detach()) == pytest.approx(st2.J, rel=1e-8)
+    np.testing.assert_allclose(gt, g * m.sigma, rtol=1e-6, atol=1e-8 * np.abs(g * m.sigma).max())
     say(f"J gpu {st2.J:.12e} direct {st.J:.12e}")
     assert st2.J == pytest.approx(st.J, rel=1e-8)
     # in the inversion variable log(sigma): dJ/dlog(sigma) = sigma dJ/dsigma (dJ/dsigma itself
