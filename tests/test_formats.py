@@ -89,3 +89,23 @@ def test_host_argument_errors_without_gpu():
     assert L.mtcocr_error_string(-3).decode() == "maximum number of iterations reached"
     assert L.mtmodel_num_edges(1, 5, 5) == -10
     assert L.mtmodel_num_edges(2, 2, 2) == 12
+
+
+def test_headers_compile_as_c99(tmp_path):
+    """include/*.h are plain C: a C99 translation unit that includes both compiles warning-free
+    (what a cgo / FFI binding of the reference side would include)."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text('#include "mtcocr.h"\n#include "mtmodel.h"\n'
+                   "int main(void) {\n"
+                   "  int (*create)(int, mtcocr_handle**) = mtcocr_create;\n"
+                   "  int64_t (*edges)(int, int, int) = mtmodel_num_edges;\n"
+                   "  (void)create;\n  (void)edges;\n  return 0;\n}\n")
+    r = subprocess.run([cc, "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        f"-I{ROOT / 'include'}", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
