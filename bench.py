@@ -118,12 +118,19 @@ def batched_only(m, freqs, args, label):
 
 
 def run_config(args):
-    """Configs 3 (four models x 32 systems) and 5 (256x256x128, 8 systems): batched FP64,
-    one JSON line in the benchmark's format."""
+    """Configs 3 (four models x 32 systems) and 5 (256x256x128, 8 systems), and config 2 in
+    batches of --max-batch systems: batched FP64, one JSON line in the benchmark's format."""
     from mtcalc_with_dfp_cocr_amd import model as M
 
     runs = []
-    if args.config == 3:
+    if args.config == 2:
+        # the headline workload in batches of at most --max-batch systems (batch-width sweep)
+        m = M.random_model(args.nx, args.ny, args.nz, seed=args.seed, h=50.0, name="bench")
+        freqs = M.frequencies(1e-3, 1e2, args.nfreq)
+        runs.append(batched_only(m, freqs, args, f"bench_batches_of_{args.max_batch}"))
+        workload = (f"batched FP64 COCR, {args.nx}x{args.ny}x{args.nz} cells, {args.nfreq} freqs "
+                    f"1e-3..1e2 Hz x 2 pol, in batches of at most {args.max_batch} systems")
+    elif args.config == 3:
         shapes = [(32, 32, 32), (48, 48, 48), (64, 64, 48), (96, 96, 64)]
         freqs = M.frequencies(1e-3, 1e3, 16)
         for k, (nx, ny, nz) in enumerate(shapes):
@@ -292,7 +299,8 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the headline (default); 3: four models x 32 systems; 5: 256x256x128")
     ap.add_argument("--max-batch", type=int, default=0,
-                    help="configs 3 and 5: solve in batches of at most this many systems (0: all)")
+                    help="solve in batches of at most this many systems (0: all); with config 2, "
+                         "only the batched FP64 solve runs (a batch-width sweep)")
     ap.add_argument("--adjoint-cells", type=int, default=4,
                     help="cells of the config-4 adjoint vs finite-difference check (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -308,7 +316,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.config != 2:
+    if args.config != 2 or args.max_batch > 0:
         run_config(args)
         return
 
