@@ -188,26 +188,44 @@ __device__ inline void block_partials(double (&v)[NV], double* partials) {
   }
 }
 
-// One-block reduction of nblk x P partials into tot[P] (fixed order).
+// One-block reduction of nblk x P partials into tot[P] (fixed order): G = kScalarThreads / P
+// threads per column each sum a strided subset, then a fixed tree over the G sums (the part
+// of G above its largest power of two folded in first).
 template <int P>
 __device__ inline void reduce_partials(const double* part, int nblk, double* tot) {
   constexpr int G = kScalarThreads / P;
+  constexpr int G2 = G >= 512 ? 512 : G >= 256 ? 256 : G >= 128 ? 128 : G >= 64 ? 64
+                   : G >= 32 ? 32 : G >= 16 ? 16 : G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
   __shared__ double tmp[kScalarThreads];
   const int t = threadIdx.x;
   const int q = t % P, j = t / P;
   double acc = 0.0;
   if (j < G) {
-    // unrolled so that the loads of one thread are in flight together
-#pragma unroll 8
-    for (int b = j; b < nblk; b += G) acc += part[(size_t)b * P + q];
+    // 16 loads of a thread in flight together, rows past nblk read as 0; one accumulator in
+    // row order
+    constexpr int U = 16;
+    for (int b0 = j; b0 < nblk; b0 += U * G) {
+      double v[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int b = b0 + k * G;
+        v[k] = b < nblk ? part[(size_t)b * P + q] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) acc += v[k];
+    }
   }
   tmp[t] = acc;
   __syncthreads();
-  if (t < P) {
-    double s = 0.0;
-    for (int jj = 0; jj < G; ++jj) s += tmp[jj * P + t];
-    tot[t] = s;
+  if (G2 < G) {
+    if (j < G - G2) tmp[t] += tmp[t + G2 * P];
+    __syncthreads();
   }
+  for (int hw = G2 / 2; hw >= 1; hw /= 2) {
+    if (j < hw) tmp[t] += tmp[t + hw * P];
+    __syncthreads();
+  }
+  if (t < P) tot[t] = tmp[t];
   __syncthreads();
 }
 
@@ -1374,24 +1392,38 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
 }
 
 // First stage of the partial-sum reduction for wide grids: workgroup g folds partial rows
-// [g*kFold, (g+1)*kFold) into out[g*P + q], in a fixed order, so that the one-workgroup scalar
-// kernels read nblk/kFold rows instead of nblk (one CU cannot stream megabytes of partials
-// quickly). Deterministic like the single-stage sum, with a different grouping.
+// [g*fold, (g+1)*fold) into out[g*P + q], in a fixed order, so that the one-workgroup scalar
+// kernels read at most about kFoldRows rows (one CU cannot stream megabytes of partials
+// quickly). fold = max(kFold, ceil(nblk / kFoldRows)) depends on the grid only, so the sums
+// are deterministic like the single-stage sum, with a different grouping.
 constexpr int kFold = 64;
+constexpr int kFoldRows = 256;
 constexpr int kFoldAbove = 2048;  // grids up to this size are reduced in one stage
 
+inline int fold_factor(int nblk) { return std::max(kFold, (nblk + kFoldRows - 1) / kFoldRows); }
+
 template <int P>
-__global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, int nblk,
+__global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, int nblk, int fold,
                                               double* __restrict__ out) {
   constexpr int R = 256 / P > 0 ? 256 / P : 1;  // partial rows handled side by side
+  constexpr int U = 16;                         // loads of a thread in flight together
   __shared__ double sh[256];
   const int g = blockIdx.x;
   const int q = threadIdx.x % P, j = threadIdx.x / P;
-  const int b0 = g * kFold, b1 = min(b0 + kFold, nblk);
+  const int b0 = g * fold, b1 = min(b0 + fold, nblk);
   double acc = 0.0;
-  if (threadIdx.x < R * P)
-#pragma unroll 4
-    for (int b = b0 + j; b < b1; b += R) acc += part[(size_t)b * P + q];
+  if (threadIdx.x < R * P) {
+    for (int bc = b0 + j; bc < b1; bc += U * R) {
+      double v[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int b = bc + k * R;
+        v[k] = b < b1 ? part[(size_t)b * P + q] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) acc += v[k];
+    }
+  }
   sh[threadIdx.x] = acc;
   __syncthreads();
   if (threadIdx.x < P) {
@@ -1857,8 +1889,9 @@ struct Engine {
       nread = nblk;
       return part;
     }
-    nread = (nblk + kFold - 1) / kFold;
-    k_fold<P><<<nread, 256, 0, st>>>(part, nblk, part2);
+    const int fold = fold_factor(nblk);
+    nread = (nblk + fold - 1) / fold;
+    k_fold<P><<<nread, 256, 0, st>>>(part, nblk, fold, part2);
     return part2;
   }
 
