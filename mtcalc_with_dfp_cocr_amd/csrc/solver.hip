@@ -1391,14 +1391,15 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
   if (threadIdx.x == 0) count_running(sc, SP, nrun);
 }
 
-// First stage of the partial-sum reduction for wide grids: workgroup g folds partial rows
+// First stage of the partial-sum reduction for wide grids (more than kFoldAbove partial
+// doubles): workgroup g folds partial rows
 // [g*fold, (g+1)*fold) into out[g*P + q], in a fixed order, so that the one-workgroup scalar
 // kernels read at most about kFoldRows rows (one CU cannot stream megabytes of partials
 // quickly). fold = max(kFold, ceil(nblk / kFoldRows)) depends on the grid only, so the sums
 // are deterministic like the single-stage sum, with a different grouping.
 constexpr int kFold = 64;
 constexpr int kFoldRows = 256;
-constexpr int kFoldAbove = 2048;  // grids up to this size are reduced in one stage
+constexpr long long kFoldAbove = 16384;  // partial doubles (rows x P) reduced in one stage
 
 inline int fold_factor(int nblk) { return std::max(kFold, (nblk + kFoldRows - 1) / kFoldRows); }
 
@@ -1885,7 +1886,7 @@ struct Engine {
   // folded by k_fold into part2 first when the grid is wide
   template <int P>
   const double* partials(hipStream_t st, int nblk, int& nread) {
-    if (nblk <= kFoldAbove) {
+    if ((long long)nblk * P <= kFoldAbove) {
       nread = nblk;
       return part;
     }

@@ -366,9 +366,10 @@ def test_bench_hooks(M, c1_model, c1_mesh):
 
 
 def test_wide_grid_two_stage_reduction(O, M, c1_model, c1_mesh):
-    """16 systems on 45000 edges: at caps above 2048 workgroups the dot-product partials are
-    folded in two stages. Same solution to rounding as the one-stage grid, deterministic, and
-    the FP64 oracle's iteration counts on two of the systems."""
+    """16 systems on 45000 edges: at 16384 workgroups the dot-product partials (more than
+    16384 doubles) are folded in two stages, at 256 they are reduced in one. Same solution to
+    rounding either way, deterministic, and the FP64 oracle's iteration counts on two of the
+    systems."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     from mtcalc_with_dfp_cocr_amd.cocr import Solver
@@ -382,7 +383,7 @@ def test_wide_grid_two_stage_reduction(O, M, c1_model, c1_mesh):
     with Solver(0) as B:
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(c1_mesh)
-        B.set_launch(1024, 1024, 1024, "rowpack")
+        B.set_launch(256, 256, 256, "rowpack")
         x1, s1 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
         B.set_launch(16384, 16384, 16384, "rowpack")
         x2, s2 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
