@@ -623,3 +623,38 @@ def test_full_size_residual(M):
     r = true_rel(1, xm[0])
     say(f"mixed f={cases[1][0]:.3g} Hz: true rel res {r:.2e}")
     assert sm[0]["req"] == 0 and r <= 3e-8
+
+
+def test_batch_with_zero_rhs_and_padding(O, M, c1_model, c1_mesh):
+    """A batch of 3 systems (one padding slot) whose middle right-hand side is zero: that
+    system retires in the prologue (X0 test, x = 0 exactly) and its lanes then run the
+    identity path of the kernels for the rest of the solve. Its x stays exactly 0, and the
+    other two match a batch of those two alone, FP64 and mixed."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("batch with a zero right-hand side and a padding slot")
+    freqs = [0.1, 1.0, 10.0]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, 1) for f in freqs])
+    prs[1] = 0.0
+    om = np.array([M.omega_of(f) for f in freqs])
+    with Solver(0) as B:
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        x3, s3 = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+        x2, s2 = B.solve_batch(om[[0, 2]], prs[[0, 2]], eps=1e-8, precision="fp64")
+        xm, sm = B.solve_batch(om, prs, eps=1e-8, precision="mixed")
+    say(f"iters {[q['iter'] for q in s3]} vs {[q['iter'] for q in s2]}")
+    assert s3[1]["req"] == -1 and not x3[1].any()
+    assert s3[0]["req"] == 0 and s3[2]["req"] == 0
+    for k, j in ((0, 0), (2, 1)):
+        assert abs(s3[k]["iter"] - s2[j]["iter"]) <= 2 + 0.02 * s2[j]["iter"]
+        assert rel(x3[k], x2[j]) <= 1e-6
+    assert sm[1]["req"] == -1 and not xm[1].any()
+    for k in (0, 2):
+        s = M.assemble(c1_model, freqs[k], 1)
+        res = M.to_complex(s.pr) - M.sparse_matrix(s) @ M.to_complex(xm[k])
+        assert sm[k]["req"] == 0
+        assert np.linalg.norm(res) / np.linalg.norm(M.to_complex(s.pr)) < 1e-8
