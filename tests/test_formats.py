@@ -89,6 +89,9 @@ def test_host_argument_errors_without_gpu():
     assert L.mtcocr_error_string(-3).decode() == "maximum number of iterations reached"
     assert L.mtmodel_num_edges(1, 5, 5) == -10
     assert L.mtmodel_num_edges(2, 2, 2) == 12
+    z3, z2, out = np.zeros(3), np.zeros(2), np.zeros(8)
+    assert L.mtcocr_sigma_gradient(None, 3, 3, 3, z3, z3, z3, 1, z2, np.zeros(108),
+                                   np.zeros(108), out) == -10
 
 
 def test_headers_compile_as_c99(tmp_path):
