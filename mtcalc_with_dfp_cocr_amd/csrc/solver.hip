@@ -2915,7 +2915,8 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 //                   blocks k_spmv_blk for 4..16 systems) x cap
 //                   (the cap also sets the partial count of its reductions)
 //   node pass       k_pc_node cap
-//   edge streams    k_pc_edge_iter cap (reported as cap_vec)
+//   edge streams    k_pc_edge_iter cap (reported as cap_vec), timed after the node pass at
+//                   its tuned cap, as it runs in an iteration
 //   update pass     k_xr_fused, tuned apart (set_launch's cap_vec sets both)
 // Only work buffers are written; the resident right-hand sides and solutions are untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
@@ -3021,6 +3022,10 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
                                                                     E.sc, E.nrun);
               } else if (fam == 2) {
+                // after the node pass (at its tuned shape), as in an iteration: the edge
+                // kernel's gathers of help then find what the node pass left in the caches
+                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
+                                                                    E.sc, E.nrun);
                 k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.w,
                                                                    E.da, E.help, E.sc, E.nrun,
                                                                    E.part);
