@@ -10,6 +10,7 @@
  *   mtcocr_set_matrix_block2x2  the SLAE arrays that main() passes to it
  *                                                                  Code/COCR_FP/HarmVect3d.cpp:196-229
  *   mtcocr_set_gradient_mesh  FoldedPreconditioner::BuildGMatrix   FoldedPreconditioner.cpp:68-259
+ *   mtcocr_build_gmatrix      the same, on the host, returning its CSR arrays
  *   mtcocr_solve              the request loop of PCOCR_2x2_Folded pcocr.cpp:61-123
  *   mtcocr_set_matrix_ks / mtcocr_solve_batch / mtcocr_set_rhs / mtcocr_solve_resident /
  *   mtcocr_get_solution       no single counterpart: they batch the reference's per-(f, pol)
@@ -31,7 +32,7 @@
  * block_2x2_solver.cpp:73-105): ig[nb+1], jg[nnzL], idi[nb+1] into di, ijg[nnzL+1] into gg,
  * 1 real per block (real coupling) or 2 reals (re, im). A = L + L^T + D, no conjugation.
  *
- * Return codes: the reference's request codes (Code/COCR_FP/rci.h:114-133) for solver outcomes,
+ * Return codes: the reference's request codes (Code/COCR_FP/rci.h:23,40-42) for solver outcomes,
  * negative MTCOCR_ERR_* for argument, device and allocation errors. Nothing throws across the
  * ABI. A handle owns one HIP stream and its device memory; do not share a handle between
  * threads.
@@ -69,7 +70,8 @@ typedef struct mtcocr_stats {
   int flags;              /* MTCOCR_FLAG_* */
   double residual_rel;    /* last MRS-smoothed relative residual (reference's residualRel) */
   double true_rel_res;    /* ||b - A x|| / ||b|| in FP64, recomputed on the device */
-  double seconds;         /* device time of DFP setup + iterations (the reference's kit span) */
+  double seconds;         /* device time of DFP setup + iterations (the reference's kit span);
+                             in a batch the whole batch's time, the same for every system */
   double seconds_io;      /* host<->device copies of b and x */
 } mtcocr_stats;
 
@@ -95,7 +97,11 @@ int mtcocr_set_matrix_ks(mtcocr_handle* h, int nb, const int* ig, const int* jg,
  * nver[kpar*14] (0-based nodes), nvkat[kpar] (0-based materials), sigma3d[materials]
  * (a node is kept when it belongs to a cell with sigma3d > 0 and is not a boundary node),
  * edges[n_edges*2] (0-based end nodes), optional hanging-node map ig_t/jg_t/gg_t (NULL when
- * n_nodes_c covers every node), is_node_bound[n_nodes]. */
+ * n_nodes_c covers every node), is_node_bound[n_nodes]. A row of G (one interior node) may
+ * hold at most 32 entries (MTCOCR_ERR_ARG, with a message, above that): 6 on a conforming
+ * hexahedral mesh, more next to hanging nodes. With a matrix set, n_edges must equal its nb;
+ * setting a matrix of another size afterwards drops the gradient (the solve then returns
+ * MTCOCR_ERR_STATE until it is set again). */
 int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_nodes_c,
                              const int* nvkat, const int* nver, const double* sigma3d,
                              const int* edges, const int* ig_t, const int* jg_t,
@@ -119,6 +125,21 @@ int mtcocr_set_mesh_order(mtcocr_handle* h, int kpar, int n_nodes, const int* nv
  * (n_edges entries). */
 int mtcocr_mesh_order(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
                       int block, int* order);
+
+/* FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259) on the
+ * host, without a device: the discrete gradient as the reference's CSR arrays, 0-based and in
+ * its node numbering (interior nodes of conductive cells, ascending), G^T (edge rows:
+ * iptr_gt[n_edges_c+1], jptr_gt[nnz], gg_gt[nnz]) and G (node rows, ascending edges:
+ * iptr_g[m+1], jptr_g[nnz], gg_g[nnz]). Arguments as mtcocr_set_gradient_mesh; hanging nodes
+ * (numbers >= n_nodes_c) spread through ig_t/jg_t/gg_t, indexed by their global number as the
+ * reference's HarmVect3d.cpp:255-301 builds them. Call once with the six output arrays NULL
+ * to get m and nnz, then with arrays of those sizes. mtcocr_set_gradient_mesh builds the same
+ * matrix and rejects (MTCOCR_ERR_ARG) one with more than 32 entries in a row of G. */
+int mtcocr_build_gmatrix(int kpar, int n_edges_c, int n_nodes_c, const int* nvkat,
+                         const int* nver, const double* sigma3d, const int* edges,
+                         const int* ig_t, const int* jg_t, const double* gg_t,
+                         const int* is_node_bound, int* m, int64_t* nnz, int* iptr_gt,
+                         int* jptr_gt, double* gg_gt, int* iptr_g, int* jptr_g, double* gg_g);
 
 /* Solve one system set by mtcocr_set_matrix_block2x2. pr = b (2*nb), x in/out. */
 int mtcocr_solve(mtcocr_handle* h, const double* pr, double* x, double eps, int maxiter,

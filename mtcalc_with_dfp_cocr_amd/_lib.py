@@ -83,6 +83,11 @@ SIGNATURES = {
     "mtcocr_set_mesh_order": (C.c_int, [C.c_void_p, C.c_int, C.c_int, i32p, C.c_int, i32p,
                                         C.c_int]),
     "mtcocr_mesh_order": (C.c_int, [C.c_int, C.c_int, i32p, C.c_int, i32p, C.c_int, i32p]),
+    "mtcocr_build_gmatrix": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, i32p, f64p, i32p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, i32p,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int64), C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
     "mtcocr_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "mtcocr_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mtcocr_get_launch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),

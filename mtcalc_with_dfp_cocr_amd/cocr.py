@@ -3,7 +3,7 @@
 `pcocr_2x2_folded` keeps the argument list, argument meaning and return codes of
 PCOCR::PCOCR_2x2_Folded (Code/COCR_FP/pcocr.h:27-31, pcocr.cpp:40-126) minus the per-thread
 scatter buffer y_omp: x is updated in place and the reference's request code is returned
-(0 REQ_OK, -1 REQ_X0_NULL, -2 REQ_DIV_BY_ZERO, -3 REQ_MAXITER, rci.h:114-133).
+(0 REQ_OK, -1 REQ_X0_NULL, -2 REQ_DIV_BY_ZERO, -3 REQ_MAXITER, rci.h:23,40-42).
 
 `Solver` is the handle-based form (one matrix + gradient resident on the GPU, many right-hand
 sides, batched frequencies sharing K and S, mixed precision).
@@ -27,6 +27,28 @@ def _i32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+MAX_BATCH = 64  # systems of one batch (the C side rejects more)
+
+
+def _vec(a, n: int, what: str) -> np.ndarray:
+    """A float64 vector of exactly n entries (the C side reads and writes n doubles)."""
+    v = _f64(a)
+    if v.ndim != 1 or v.size != n:
+        raise ValueError(f"{what} must be a vector of {n} doubles (2 * nb), got shape {v.shape}")
+    return v
+
+
+def _batch(a, nb: int, what: str, nsys: int | None = None) -> np.ndarray:
+    """nsys x 2*nb float64 right-hand sides or solutions (1 <= nsys <= 64)."""
+    v = _f64(np.atleast_2d(a))
+    if v.ndim != 2 or v.shape[1] != 2 * nb or not 1 <= v.shape[0] <= MAX_BATCH:
+        raise ValueError(f"{what} must have shape (nsys, {2 * nb}) with 1 <= nsys <= "
+                         f"{MAX_BATCH}, got {v.shape}")
+    if nsys is not None and v.shape[0] != nsys:
+        raise ValueError(f"{what} must hold {nsys} systems, got {v.shape[0]}")
+    return v
 
 
 class Solver:
@@ -87,9 +109,11 @@ class Solver:
     def solve(self, pr, x0=None, eps: float = 1e-8, maxiter: int = 10000,
               precision: str = "fp64"):
         """One system set by set_matrix. Returns (x, stats dict)."""
-        x = _f64(np.zeros(2 * self.nb) if x0 is None else x0).copy()
+        n = 2 * self.nb
+        pr = _vec(pr, n, "pr")
+        x = (np.zeros(n) if x0 is None else _vec(x0, n, "x0")).copy()
         st = MtcocrStats()
-        rc = self._L.mtcocr_solve(self._h, _f64(pr), x, float(eps), int(maxiter),
+        rc = self._L.mtcocr_solve(self._h, pr, x, float(eps), int(maxiter),
                                   PRECISION[precision], C.byref(st))
         check(rc, "mtcocr_solve")
         return x, st.as_dict()
@@ -97,10 +121,10 @@ class Solver:
     def solve_batch(self, omega, prs, x0=None, eps: float = 1e-8, maxiter: int = 10000,
                     precision: str = "fp64"):
         """nsys systems A(omega_s) x_s = b_s on the K/S matrix. prs: (nsys, 2*nb)."""
-        prs = _f64(prs)
+        prs = _batch(prs, self.nb, "prs")
         nsys = prs.shape[0]
-        x = _f64(np.zeros_like(prs) if x0 is None else x0).copy()
-        om = None if omega is None else _f64(omega)
+        x = (np.zeros_like(prs) if x0 is None else _batch(x0, self.nb, "x0", nsys)).copy()
+        om = None if omega is None else _vec(omega, nsys, "omega")
         sts = (MtcocrStats * nsys)()
         rc = self._L.mtcocr_solve_batch(self._h, nsys, ptr(om), prs.ravel(), x.ravel(),
                                         float(eps), int(maxiter), PRECISION[precision], sts)
@@ -108,12 +132,13 @@ class Solver:
         return x.reshape(prs.shape), [s.as_dict() for s in sts]
 
     def set_rhs(self, prs, omega=None, x0=None):
-        prs = _f64(np.atleast_2d(prs))
-        self.nsys = prs.shape[0]
-        om = None if omega is None else _f64(omega)
-        x0 = None if x0 is None else _f64(x0)
-        check(self._L.mtcocr_set_rhs(self._h, self.nsys, ptr(om), prs.ravel(), ptr(x0)),
+        prs = _batch(prs, self.nb, "prs")
+        nsys = prs.shape[0]
+        om = None if omega is None else _vec(omega, nsys, "omega")
+        x0 = None if x0 is None else _batch(x0, self.nb, "x0", nsys)
+        check(self._L.mtcocr_set_rhs(self._h, nsys, ptr(om), prs.ravel(), ptr(x0)),
               "mtcocr_set_rhs")
+        self.nsys = nsys
 
     def solve_resident(self, eps: float = 1e-8, maxiter: int = 10000, precision: str = "fp64"):
         sts = (MtcocrStats * self.nsys)()
@@ -180,7 +205,7 @@ class Solver:
         """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""
         y = np.empty(2 * self.nb)
         check(self._L.mtcocr_apply(self._h, {"spmv": 0, "precond": 1}[op], PRECISION[precision],
-                                   _f64(x), y), "mtcocr_apply")
+                                   _vec(x, 2 * self.nb, "x"), y), "mtcocr_apply")
         return y
 
     def sigma_gradient(self, shape_nodes, x, y, z, coef, u, v) -> np.ndarray:
@@ -208,6 +233,30 @@ class Solver:
                 "device_bytes": d.value}
 
 
+def build_gmatrix(ma: MeshArrays, ig_t=None, jg_t=None, gg_t=None, n_nodes_c: int | None = None):
+    """FoldedPreconditioner::BuildGMatrix (FoldedPreconditioner.cpp:68-259) on the host: the
+    CSR arrays (iptr_gt, jptr_gt, gg_gt) of G^T and (iptr_g, jptr_g, gg_g) of G, 0-based, in
+    the reference's node numbering. n_nodes_c: conforming nodes (default: all of the mesh's);
+    nodes above it are hanging and spread through the T-matrix ig_t/jg_t/gg_t (indexed by the
+    hanging node's global number)."""
+    L = lib()
+    n = len(ma.edges) // 2
+    nc = int(ma.kuzlov if n_nodes_c is None else n_nodes_c)
+    t = (ptr(None if ig_t is None else _i32(ig_t)), ptr(None if jg_t is None else _i32(jg_t)),
+         ptr(None if gg_t is None else _f64(gg_t)))
+    args = (int(ma.kpar), n, nc, _i32(ma.nvkat), _i32(ma.nver), _f64(ma.sig_flags),
+            _i32(ma.edges), *t, _i32(ma.is_node_bound))
+    m, nnz = C.c_int(), C.c_int64()
+    check(L.mtcocr_build_gmatrix(*args, C.byref(m), C.byref(nnz), None, None, None, None, None,
+                                 None), "mtcocr_build_gmatrix")
+    it, jt, gt = np.empty(n + 1, np.int32), np.empty(nnz.value, np.int32), np.empty(nnz.value)
+    ip, jp, gp = np.empty(m.value + 1, np.int32), np.empty(nnz.value, np.int32), np.empty(nnz.value)
+    keep = [a.ctypes.data_as(C.c_void_p) for a in (it, jt, gt, ip, jp, gp)]
+    check(L.mtcocr_build_gmatrix(*args, C.byref(m), C.byref(nnz), *keep),
+          "mtcocr_build_gmatrix")
+    return (it, jt, gt), (ip, jp, gp)
+
+
 def mesh_order(ma: MeshArrays, block: int = 16) -> np.ndarray:
     """The device's locality order of a structured hexahedral mesh: order[i] = edge index of
     device row i (pencils of block x block nodes swept along z)."""
@@ -225,6 +274,9 @@ def pcocr_2x2_folded(n, ig, jg, idi, ijg, di, gg, pr, x, eps, maxiter, kpar, n_e
     solution; returns the reference's request code."""
     if x.dtype != np.float64 or not x.flags.c_contiguous:
         raise TypeError("x must be a C-contiguous float64 array (it is updated in place)")
+    if int(n) <= 0 or x.size != int(n):
+        raise ValueError(f"x must hold n = {int(n)} doubles, got {x.size}")
+    pr = _vec(pr, int(n), "pr")
     st = MtcocrStats()
     xyz_a = None if xyz is None else _f64(xyz)
     rc = lib().mtcocr_pcocr_2x2_folded(

@@ -214,12 +214,14 @@ bool read_inputs(const std::string& dir, Inputs& in, std::ofstream& log, bool ne
   return true;
 }
 
+// kit (WriteKitChrono, base_solver.cpp:181-200) and v3.dat (HarmVect3d.cpp:336); `seconds` is
+// the solve time written to kit
 bool write_outputs(const std::string& dir, const Inputs& in, const std::vector<double>& x,
-                   const mtcocr_stats& st) {
+                   const mtcocr_stats& st, double seconds) {
   FILE* f = std::fopen(join(dir, "kit").c_str(), "w");
   if (f) {
     std::fprintf(f, "r=%e\teps=%e\tkit=%d\t", st.residual_rel, in.eps, st.iter);
-    std::fprintf(f, "time=%lf sec\n", st.seconds);
+    std::fprintf(f, "time=%lf sec\n", seconds);
     std::fclose(f);
   }
   f = std::fopen(join(dir, "v3.dat").c_str(), "wb");
@@ -290,7 +292,7 @@ int run_single(const std::string& dir, bool mixed, bool strict) {
   }
   log << st.iter << "    " << std::scientific << st.residual_rel << std::endl;
   stamp("Writing result");
-  if (!write_outputs(dir, in, u, st)) return 1;
+  if (!write_outputs(dir, in, u, st, st.seconds)) return 1;
   stamp("All done");
   std::printf("time=%lf\n", st.seconds);
   if (strict && rc != 0) return 2;
@@ -312,11 +314,20 @@ int run_batch(const std::vector<std::string>& dirs, bool mixed, bool strict) {
   }
   const Inputs& a = ins[0];
   const int nb = a.n_c;
-  for (int s = 1; s < nsys; ++s)
-    if (ins[s].n_c != nb || ins[s].jg != a.jg || ins[s].ijg != a.ijg || ins[s].idi != a.idi) {
+  for (int s = 1; s < nsys; ++s) {
+    if (ins[s].n_c != nb || ins[s].ig != a.ig || ins[s].jg != a.jg || ins[s].ijg != a.ijg ||
+        ins[s].idi != a.idi) {
       std::fprintf(stderr, "COCR_FP: --batch directories do not share one matrix portrait\n");
       return 1;
     }
+    // one batch runs with one tolerance and iteration limit (kuslau)
+    if (ins[s].eps != a.eps || ins[s].maxiter != a.maxiter) {
+      std::fprintf(stderr, "COCR_FP: --batch: %s has eps/maxiter %g/%d, %s has %g/%d\n",
+                   dirs[s].c_str(), ins[s].eps, ins[s].maxiter, dirs[0].c_str(), a.eps,
+                   a.maxiter);
+      return 1;
+    }
+  }
   // split the first directory's matrix into K and S; verify every other one against it
   const double w0 = 2.0 * kPi * a.nu;
   const int nnz = a.ig[nb];
@@ -331,19 +342,28 @@ int run_batch(const std::vector<std::string>& dirs, bool mixed, bool strict) {
     kg[k] = a.gg[a.ijg[k]];
     sg[k] = sz == 2 ? a.gg[a.ijg[k] + 1] / w0 : 0.0;
   }
+  // every block of every directory (diagonal and off-diagonal) must be K + i*omega_s*S
+  auto matches = [](double re, double im, double k, double s_om) {
+    return std::fabs(re - k) <= 1e-12 * std::fabs(k) + 1e-300 &&
+           std::fabs(im - s_om) <= 1e-10 * std::fabs(im) + 1e-300;
+  };
   std::vector<double> omega(nsys);
   for (int s = 0; s < nsys; ++s) {
     omega[s] = 2.0 * kPi * ins[s].nu;
     const Inputs& b = ins[s];
-    for (int i = 0; i < nb; ++i) {
+    bool ok = true;
+    for (int i = 0; ok && i < nb; ++i) {
       const int sz = b.idi[i + 1] - b.idi[i];
-      const double re = b.di[b.idi[i]], im = sz == 2 ? b.di[b.idi[i] + 1] : 0.0;
-      if (std::fabs(re - kd[i]) > 1e-12 * std::fabs(kd[i]) + 1e-300 ||
-          std::fabs(im - omega[s] * sd[i]) > 1e-10 * std::fabs(im) + 1e-300) {
-        std::fprintf(stderr, "COCR_FP: --batch: %s is not K + i*omega*S of %s\n", dirs[s].c_str(),
-                     dirs[0].c_str());
-        return 1;
-      }
+      ok = matches(b.di[b.idi[i]], sz == 2 ? b.di[b.idi[i] + 1] : 0.0, kd[i], omega[s] * sd[i]);
+    }
+    for (int k = 0; ok && k < nnz; ++k) {
+      const int sz = b.ijg[k + 1] - b.ijg[k];
+      ok = matches(b.gg[b.ijg[k]], sz == 2 ? b.gg[b.ijg[k] + 1] : 0.0, kg[k], omega[s] * sg[k]);
+    }
+    if (!ok) {
+      std::fprintf(stderr, "COCR_FP: --batch: %s is not K + i*omega*S of %s\n", dirs[s].c_str(),
+                   dirs[0].c_str());
+      return 1;
     }
   }
   int dev = 0;
@@ -369,11 +389,17 @@ int run_batch(const std::vector<std::string>& dirs, bool mixed, bool strict) {
     std::fprintf(stderr, "COCR_FP: %s\n", mtcocr_error_string(rc));
     return 1;
   }
+  // The systems of a batch share every iteration's launches, so a system has no time of its
+  // own: each kit gets the batch's device time divided by the number of systems (its share),
+  // and logharm3dCalc the whole batch's time.
+  const double batch_s = nsys > 0 ? st[0].seconds : 0.0;
   for (int s = 0; s < nsys; ++s) {
     std::vector<double> u(x.begin() + (size_t)s * 2 * nb, x.begin() + (size_t)(s + 1) * 2 * nb);
     logs[s] << st[s].iter << "    " << std::scientific << st[s].residual_rel << std::endl;
-    if (!write_outputs(dirs[s], ins[s], u, st[s])) return 1;
+    logs[s] << "batch of " << nsys << " systems: " << std::fixed << batch_s << " sec" << std::endl;
+    if (!write_outputs(dirs[s], ins[s], u, st[s], batch_s / nsys)) return 1;
   }
+  std::printf("batch time=%lf sec for %d systems\n", batch_s, nsys);
   if (strict && rc != 0) return 2;
   return 0;
 }

@@ -264,10 +264,9 @@ int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const 
   return build_rows(nb, ig, jg, lower, diag, order, out);
 }
 
-int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
-                   const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
-                   const double* gg_t, const int* is_node_bound, const int* edge_order,
-                   GradientEll& out) {
+int gradient_csr(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
+                 const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
+                 const double* gg_t, const int* is_node_bound, GradientCsr& out) {
   if (kpar <= 0 || n_edges <= 0 || n_nodes_c <= 0 || !nvkat || !nver || !sigma3d || !edges ||
       !is_node_bound)
     return MTCOCR_ERR_ARG;
@@ -289,49 +288,67 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
   for (int i = 0; i < n_nodes_c; ++i)
     if (in_sigma[i]) renum[i] = m++;
   // G^T rows (edge -> nodes): -1/2 at the first node, +1/2 at the second; hanging nodes
-  // (index >= n_nodes_c) spread through the T-matrix ig_t/jg_t/gg_t
-  std::vector<int64_t> tptr(n_edges + 1, 0);
-  std::vector<int> tcol;
-  std::vector<double> tval;
-  tcol.reserve((size_t)n_edges * 2);
-  tval.reserve((size_t)n_edges * 2);
+  // (index >= n_nodes_c) spread through the T-matrix ig_t/jg_t/gg_t onto conforming nodes
+  out.m = m;
+  out.tptr.assign(n_edges + 1, 0);
+  out.tcol.clear();
+  out.tval.clear();
+  out.tcol.reserve((size_t)n_edges * 2);
+  out.tval.reserve((size_t)n_edges * 2);
   for (int i = 0; i < n_edges; ++i) {
     for (int k = 0; k < 2; ++k) {
       const int v = edges[(int64_t)i * 2 + k];
       const double sgn = (k == 0) ? -0.5 : 0.5;
+      if (v < 0) return MTCOCR_ERR_ARG;
       if (v < n_nodes_c) {
-        if (v >= 0 && in_sigma[v]) {
-          tcol.push_back(renum[v]);
-          tval.push_back(sgn);
+        if (in_sigma[v]) {
+          out.tcol.push_back(renum[v]);
+          out.tval.push_back(sgn);
         }
       } else {
         if (!ig_t || !jg_t || !gg_t) return MTCOCR_ERR_ARG;
         for (int ii = ig_t[v]; ii < ig_t[v + 1]; ++ii) {
           const int kk = jg_t[ii];
+          if (kk < 0 || kk >= n_nodes_c) return MTCOCR_ERR_ARG;  // T rows hold conforming nodes
           if (in_sigma[kk]) {
-            tcol.push_back(renum[kk]);
-            tval.push_back(sgn * gg_t[ii]);
+            out.tcol.push_back(renum[kk]);
+            out.tval.push_back(sgn * gg_t[ii]);
           }
         }
       }
     }
-    tptr[i + 1] = (int64_t)tcol.size();
+    out.tptr[i + 1] = (int64_t)out.tcol.size();
   }
-  // G rows (node -> edges, ascending edge)
+  // G rows (node -> edges, ascending edge): the transpose
   std::vector<int64_t> gcnt(m + 1, 0);
-  for (size_t k = 0; k < tcol.size(); ++k) gcnt[tcol[k]]++;
-  std::vector<int64_t> gptr(m + 1, 0);
-  for (int i = 0; i < m; ++i) gptr[i + 1] = gptr[i] + gcnt[i];
-  std::vector<int> gc(tcol.size());
-  std::vector<double> gv(tcol.size());
-  std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+  for (size_t k = 0; k < out.tcol.size(); ++k) gcnt[out.tcol[k]]++;
+  out.gptr.assign(m + 1, 0);
+  for (int i = 0; i < m; ++i) out.gptr[i + 1] = out.gptr[i] + gcnt[i];
+  out.gcol.assign(out.tcol.size(), 0);
+  out.gval.assign(out.tcol.size(), 0.0);
+  std::vector<int64_t> fill(out.gptr.begin(), out.gptr.end() - 1);
   for (int e = 0; e < n_edges; ++e)
-    for (int64_t k = tptr[e]; k < tptr[e + 1]; ++k) {
-      const int node = tcol[k];
-      gc[fill[node]] = e;
-      gv[fill[node]] = tval[k];
+    for (int64_t k = out.tptr[e]; k < out.tptr[e + 1]; ++k) {
+      const int node = out.tcol[k];
+      out.gcol[fill[node]] = e;
+      out.gval[fill[node]] = out.tval[k];
       fill[node]++;
     }
+  return 0;
+}
+
+int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
+                   const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
+                   const double* gg_t, const int* is_node_bound, const int* edge_order,
+                   GradientEll& out) {
+  GradientCsr C;
+  const int rc = gradient_csr(kpar, n_edges, n_nodes_c, nvkat, nver, sigma3d, edges, ig_t, jg_t,
+                              gg_t, is_node_bound, C);
+  if (rc) return rc;
+  const int m = C.m;
+  const std::vector<int64_t>&tptr = C.tptr, &gptr = C.gptr;
+  const std::vector<int>&tcol = C.tcol, &gc = C.gcol;
+  const std::vector<double>&tval = C.tval, &gv = C.gval;
   // Device order: edges as the matrix rows (edge_order, new -> old); interior nodes ranked by
   // their first incident edge in that order, so that a node's edges and the node sit close.
   // Entry order inside every row of G and G^T stays the reference's.
@@ -478,5 +495,35 @@ extern "C" int mtcocr_mesh_order(int kpar, int n_nodes, const int* nver, int n_e
   const int rc = mtcocr::order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
   if (rc) return rc;
   std::memcpy(order, ord.data(), sizeof(int) * (size_t)n_edges);
+  return 0;
+}
+
+// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259) on the host:
+// the same CSR arrays (0-based), G^T first, in the reference's node numbering and entry order.
+extern "C" int mtcocr_build_gmatrix(int kpar, int n_edges_c, int n_nodes_c, const int* nvkat,
+                                    const int* nver, const double* sigma3d, const int* edges,
+                                    const int* ig_t, const int* jg_t, const double* gg_t,
+                                    const int* is_node_bound, int* m, int64_t* nnz,
+                                    int* iptr_gt, int* jptr_gt, double* gg_gt, int* iptr_g,
+                                    int* jptr_g, double* gg_g) {
+  if (!m || !nnz) return MTCOCR_ERR_ARG;
+  mtcocr::GradientCsr C;
+  const int rc = mtcocr::gradient_csr(kpar, n_edges_c, n_nodes_c, nvkat, nver, sigma3d, edges,
+                                      ig_t, jg_t, gg_t, is_node_bound, C);
+  if (rc) return rc;
+  *m = C.m;
+  *nnz = (int64_t)C.tcol.size();
+  if (*nnz > 2147483647LL) return MTCOCR_ERR_ARG;
+  const bool fill_t = iptr_gt && jptr_gt && gg_gt, fill_g = iptr_g && jptr_g && gg_g;
+  if (fill_t) {
+    for (int e = 0; e <= n_edges_c; ++e) iptr_gt[e] = int(C.tptr[e]);
+    std::memcpy(jptr_gt, C.tcol.data(), sizeof(int) * C.tcol.size());
+    std::memcpy(gg_gt, C.tval.data(), sizeof(double) * C.tval.size());
+  }
+  if (fill_g) {
+    for (int i = 0; i <= C.m; ++i) iptr_g[i] = int(C.gptr[i]);
+    std::memcpy(jptr_g, C.gcol.data(), sizeof(int) * C.gcol.size());
+    std::memcpy(gg_g, C.gval.data(), sizeof(double) * C.gval.size());
+  }
   return 0;
 }

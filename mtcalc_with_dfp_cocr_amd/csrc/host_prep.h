@@ -63,7 +63,20 @@ int ell_from_block2x2(int nb, const int* ig, const int* jg, const int* idi, cons
 int ell_from_ks(int nb, const int* ig, const int* jg, const double* k_di, const double* k_gg,
                 const double* s_di, const double* s_gg, const int* order, SlicedEll& out);
 
-// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259), with the
+// FoldedPreconditioner::BuildGMatrix (Code/COCR_FP/FoldedPreconditioner.cpp:68-259) as CSR in
+// the reference's numbering: G^T (edge rows, tptr/tcol/tval) and G (interior-node rows,
+// gptr/gcol/gval, ascending edges).
+struct GradientCsr {
+  int m = 0;
+  std::vector<int64_t> tptr, gptr;
+  std::vector<int> tcol, gcol;
+  std::vector<double> tval, gval;
+};
+int gradient_csr(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,
+                 const double* sigma3d, const int* edges, const int* ig_t, const int* jg_t,
+                 const double* gg_t, const int* is_node_bound, GradientCsr& out);
+
+// The same gradient in the device's padded layout, with the
 // edges in `edge_order` (new -> old, may be null) and the interior nodes ranked by their first
 // incident edge in that order.
 int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const int* nver,

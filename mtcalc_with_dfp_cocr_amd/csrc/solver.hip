@@ -115,6 +115,10 @@ struct BlkView {
 };
 
 constexpr int kRowBlock = 16;       // rows per block of the halo-staged SpMV
+// Longest row of G (entries of one interior node) the preconditioner set-up takes. A node of a
+// conforming hexahedral mesh has at most 6 edges; hanging nodes of a non-conforming mesh add
+// the edges of their T-matrix rows (a 2:1 face adds up to 12 more).
+constexpr int kMaxGRow = 32;
 constexpr int kHaloStage = 192;     // largest block halo it stages (columns)
 
 template <class T>
@@ -396,10 +400,10 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
     const int i = int(t / SP);
     const int s = int(t % SP);
     const double w = omega[s];
-    int ecol[8];
-    double eval[8];
+    int ecol[kMaxGRow];
+    double eval[kMaxGRow];
     int wg = 0;
-    for (int k = 0; k < G.wg && k < 8; ++k) {
+    for (int k = 0; k < G.wg && k < kMaxGRow; ++k) {
       const double g = G.gval[(size_t)k * G.m + i];
       if (g == 0.0) continue;  // padding of shorter rows
       ecol[wg] = G.gcol[(size_t)k * G.m + i];
@@ -1634,9 +1638,9 @@ struct mtcocr_handle {
   DevBuf rp, rb, pcol, pa, pb;  // row pack of the off-diagonal entries
   DevBuf hoff, halo, lcol, ldiag;  // 16-row blocks of the row pack and their halos
   int nblk = 0, halo_max = 0;      // halo_max 0: no row blocks
-  // gradient
+  // gradient (g_nb: the edge count it was built for; the solver runs only when it equals nb)
   bool have_g = false;
-  int m = 0, wg = 0, wgt = 0;
+  int m = 0, wg = 0, wgt = 0, g_nb = 0;
   DevBuf gcol, gtcol, gval64, gtval64, gval32, gtval32;
 
   // batch state
@@ -2506,6 +2510,8 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
   h->nblk = E.hoff.empty() ? 0 : int(E.hoff.size()) - 1;
   HIPC(hipStreamSynchronize(st));
   h->have_matrix = true;
+  // a gradient built for another edge count does not fit this matrix: set it again
+  if (h->have_g && h->g_nb != h->nb) h->have_g = false;
   h->nsys = 0;
   h->work_prec = -1;
   return 0;
@@ -2603,7 +2609,12 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
   int rc = build_gradient(kpar, n_edges, n_nodes_c, nvkat, nver, sigma3d, edges, ig_t, jg_t, gg_t,
                           is_node_bound, h->order.empty() ? nullptr : h->order.data(), G);
   if (rc) return rc;
-  if (G.wg > 8) return MTCOCR_ERR_ARG;  // more than 8 edges at a node: not a hexahedral mesh
+  if (G.wg > kMaxGRow) {
+    std::snprintf(g_err, sizeof(g_err),
+                  "a node of G has %d entries; the preconditioner set-up takes at most %d", G.wg,
+                  kMaxGRow);
+    return MTCOCR_ERR_ARG;
+  }
   hipStream_t st = h->stream;
   h->have_g = false;
   if ((rc = upload(h, h->gcol, G.gcol, st)) || (rc = upload(h, h->gtcol, G.gtcol, st)) ||
@@ -2616,6 +2627,7 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
   h->m = G.m;
   h->wg = G.wg;
   h->wgt = G.wgt;
+  h->g_nb = n_edges;
   h->have_g = true;
   h->work_prec = -1;
   return 0;
@@ -2656,7 +2668,7 @@ int mtcocr_set_rhs(mtcocr_handle* h, int nsys, const double* omega, const double
 int mtcocr_solve_resident(mtcocr_handle* h, double eps, int maxiter, int precision,
                           mtcocr_stats* stats) {
   if (!h || !stats || maxiter < 0 || !(eps > 0)) return MTCOCR_ERR_ARG;
-  if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
+  if (!h->have_matrix || !h->have_g || h->g_nb != h->nb || h->nsys < 1) return MTCOCR_ERR_STATE;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
   HIPC(hipSetDevice(h->device));
   int rc = with_sp(h->sp, [&](auto spc) -> int {
@@ -2923,7 +2935,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                     int* cap_node, int* spmv_layout, double* spmv_ms) {
   if (!h || reps < 1) return MTCOCR_ERR_ARG;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
-  if (!h->have_matrix || !h->have_g || h->nsys < 1) return MTCOCR_ERR_STATE;
+  if (!h->have_matrix || !h->have_g || h->g_nb != h->nb || h->nsys < 1) return MTCOCR_ERR_STATE;
   HIPC(hipSetDevice(h->device));
   // FP64 batches wider than kGroupWidth iterate in groups of kGroupWidth: tune that width
   const int sp =
@@ -3079,7 +3091,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
 int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, double* y) {
   if (!h || !x || !y || (op != 0 && op != 1)) return MTCOCR_ERR_ARG;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
-  if (!h->have_matrix || (op == 1 && !h->have_g)) return MTCOCR_ERR_STATE;
+  if (!h->have_matrix || (op == 1 && (!h->have_g || h->g_nb != h->nb))) return MTCOCR_ERR_STATE;
   HIPC(hipSetDevice(h->device));
   int rc;
   if (h->nsys < 1) {

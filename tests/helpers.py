@@ -34,11 +34,12 @@ def dense_preconditioner(A, G):
     return apply, da, dn
 
 
-def textbook_pcocr(A, minv, b, iters):
+def textbook_pcocr(A, minv, b, iters, residuals=False):
     """Preconditioned COCR for complex-symmetric A (unconjugated bilinear forms):
     u = M^-1 r, z = A u, p = u, q = z;  alpha = (u,z)/(q, M^-1 q);  x += alpha p;
     r -= alpha q;  u -= alpha M^-1 q;  beta = (u,Au)_new/(u,Au)_old;  p = u + beta p;
-    q = Au + beta q.  Returns the iterates x_1..x_iters."""
+    q = Au + beta q.  Returns the iterates x_1..x_iters (and, with residuals=True, also the
+    recurrence residuals r_0..r_iters)."""
     x = np.zeros_like(b)
     r = b.copy()
     u = minv(r)
@@ -46,6 +47,7 @@ def textbook_pcocr(A, minv, b, iters):
     p, q = u.copy(), z.copy()
     uz = u @ z
     xs = []
+    rs = [r.copy()]
     for _ in range(iters):
         mq = minv(q)
         alpha = uz / (q @ mq)
@@ -59,7 +61,33 @@ def textbook_pcocr(A, minv, b, iters):
         p = u + beta * p
         q = z + beta * q
         xs.append(x.copy())
-    return xs
+        rs.append(r.copy())
+    return (xs, rs) if residuals else xs
+
+
+def mrs_residuals(rs, clamp=True, copy=True):
+    """Minimal residual smoothing of a residual sequence r_0, r_1, ... taken as real vectors
+    (re and im parts side by side): s_0 = r_0; for k >= 1, u = r_k - s, w = -(s.u)/(u.u)
+    (0 when u.u < 1e-100); w < 0 leaves s unchanged (clamp), w > 1 restarts s = r_k (copy),
+    otherwise s += w u. Returns the norms ||s_k|| and the unclamped weights w_k. The flags
+    switch the two clamps off, to show that a check against this sequence sees them."""
+    real = lambda v: np.concatenate([v.real, v.imag])
+    s = real(rs[0])
+    norms, ws = [np.linalg.norm(s)], []
+    for rk in rs[1:]:
+        r = real(rk)
+        u = r - s
+        uu = u @ u
+        w = 0.0 if abs(uu) < 1e-100 else -(s @ u) / uu
+        ws.append(w)
+        if clamp and w < 0:
+            pass
+        elif copy and w > 1:
+            s = r.copy()
+        else:
+            s = s + w * u
+        norms.append(np.linalg.norm(s))
+    return np.array(norms), np.array(ws)
 
 
 def identity_system(M):

@@ -7,7 +7,8 @@ brute-force construction, an independent textbook recurrence and the smoothing i
 import numpy as np
 import pytest
 
-from helpers import brute_force_g, dense_preconditioner, identity_system, textbook_pcocr
+from helpers import (brute_force_g, dense_preconditioner, identity_system, mrs_residuals,
+                     textbook_pcocr)
 
 
 @pytest.fixture(scope="module")
@@ -134,6 +135,39 @@ def test_mrs_invariant_and_monotone(O, M, c1_system, c1_mesh):
     # the returned x is the raw COCR iterate (not y): its residual differs from the MRS one
     rel_x = np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b)
     assert rel_x < 5e-8 and rel_x != pytest.approx(rel_y, rel=1e-6)
+
+
+def test_mrs_clamps_follow_textbook_residuals(O, M):
+    """The oracle's stop-test residual (MRS.cpp:85-153: w < 0 skips the step, w > 1 restarts
+    from the COCR residual) against minimal residual smoothing applied in numpy to the
+    residuals of the independent textbook recurrence. On this air-free model at 100 Hz both
+    clamps fire within 30 iterations (w < 0 twice, w > 1 twice), and the oracle's trace follows
+    the clamped sequence to 1e-9; the sequences without the skip or without the restart miss
+    it by more than 1e-3, so an oracle that dropped either clamp fails here."""
+    m = M.random_model(6, 6, 6, seed=3, h=100.0, n_pad=2, n_air=0, name="earth2")
+    s = M.assemble(m, 100.0, 1)
+    ma = M.mesh_arrays(m)
+    A = M.sparse_matrix(s)
+    apply, _, _ = dense_preconditioner(A, brute_force_g(ma, s.nb))
+    b = M.to_complex(s.pr)
+    K = 30
+    _, rs = textbook_pcocr(A, apply, b, K, residuals=True)
+    norms, ws = mrs_residuals(rs)
+    assert (ws < 0).sum() >= 2 and (ws > 1).sum() >= 2
+    _, st, tr = O.pcocr(s, ma, eps=1e-30, maxiter=K, trace=True)
+    assert st["req"] == -3
+    bn = np.linalg.norm(b)
+    want = norms[1:] / bn
+    got = tr[:K]
+    assert np.abs(got - want).max() <= 1e-9 * want.min()
+    for kw in ({"clamp": False}, {"copy": False}):
+        alt = mrs_residuals(rs, **kw)[0][1:] / bn
+        assert np.abs(alt - want).max() > 1e-3 * want.min()
+    # the stop point: the first iteration whose smoothed residual is below eps
+    _, rs2 = textbook_pcocr(A, apply, b, 46, residuals=True)
+    rel = mrs_residuals(rs2)[0][1:] / bn
+    _, st2, _ = O.pcocr(s, ma, eps=1e-8, maxiter=1000)
+    assert st2["req"] == 0 and st2["iter"] == int(np.argmax(rel < 1e-8)) + 1
 
 
 def test_special_cases(O, M, tiny):
