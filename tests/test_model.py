@@ -256,6 +256,56 @@ def test_harm1d_halfspace_analytic(M, nu, sig):
     assert z[0] == -1e7 and z[-1] == 0.0 and np.all(np.diff(z) > 0)
 
 
+def layered_recursion(tops, sig, nu):
+    """Closed-form 1-D MT response of a layered earth (the standard impedance recursion from
+    the half-space up): with A = q (1/mu) A' and q continuous across interfaces, a layer of
+    k = sqrt(i w mu sigma), zeta = mu / k and thickness h maps q_bottom to
+    q_top = zeta (q_bottom + zeta tanh(k h)) / (zeta + q_bottom tanh(k h)), and A itself by
+    A_top / A_bottom = cosh(k h) + (zeta / q_bottom) sinh(k h). Layers as the model keeps them
+    (deepest first, the last top at 0). Returns A(0) for unit surface flux and the A values at
+    every interface below the surface (top to bottom)."""
+    w = 2 * np.pi * nu
+    mu = 4e-7 * np.pi
+    t, s = list(tops[::-1]), list(sig[::-1])  # top layer first; t[0] = 0
+    k = np.sqrt(1j * w * mu * np.array(s))
+    zeta = mu / k
+    qs = [zeta[-1]]  # q at the top of each layer, bottom up
+    for j in range(len(s) - 2, -1, -1):
+        th = np.tanh(k[j] * (t[j] - t[j + 1]))
+        qb = qs[-1]
+        qs.append(zeta[j] * (qb + zeta[j] * th) / (zeta[j] + qb * th))
+    qs = qs[::-1]  # q at the top of layer j, top first
+    a = [qs[0]]  # A at the top of layer j (unit flux at z = 0)
+    for j in range(len(s) - 1):
+        h = t[j] - t[j + 1]
+        a.append(a[-1] / (np.cosh(k[j] * h) + zeta[j] / qs[j + 1] * np.sinh(k[j] * h)))
+    return a[0], np.array(a[1:])
+
+
+@pytest.mark.parametrize("nu", [1e-3, 1e-2, 0.1, 1.0, 10.0, 100.0, 1e3])
+def test_harm1d_layered_earth(M, nu):
+    """Harm1D on config 1's three layers (100, 10 and 1000 Ohm*m; interfaces at 500 m and, on
+    the mesh, 1610.4 m) against the closed-form recursion: A at the surface (E = -i w A(0), so
+    the normal-field impedance) and at both interfaces within 1e-4 relative. A layer misplaced
+    by 100 m, or the top layer taken as a half-space, moves the surface value by more than
+    1e-3 at every frequency here, so a wrong layer term fails."""
+    m = M.config1_model()
+    z, us, uc = M.harm1d(m, nu)
+    a = us + 1j * uc
+    a0, ai = layered_recursion(m.layer_top, m.layer_sigma, nu)
+    assert abs(a[-1] - a0) <= 1e-4 * abs(a0)
+    for zi, want in zip(m.layer_top[::-1][1:], ai):
+        k = int(np.argmin(np.abs(z - zi)))
+        assert abs(z[k] - zi) <= 1e-9 * abs(zi)  # interfaces are nodes of the 1-D mesh
+        got = a[k]
+        assert abs(got - want) <= 1e-4 * abs(a0)
+    wrong = m.layer_top.copy()
+    wrong[1] -= 100.0
+    assert abs(layered_recursion(wrong, m.layer_sigma, nu)[0] - a0) > 1e-3 * abs(a0)
+    half = layered_recursion(m.layer_top[-1:], m.layer_sigma[-1:], nu)[0]
+    assert abs(half - a0) > 1e-3 * abs(a0)
+
+
 def test_harm1d_mesh_rule(M):
     """1-D nodes: steps of 1e-2 m growing x1.02 from the surface, layer interfaces inserted,
     the last node at -1e7 (Mesh_1d_mtz::Gen_1d_mesh)."""
