@@ -235,6 +235,23 @@ int mtcocr_bench_copy(mtcocr_handle* h, int64_t nbytes, int reps, double* avg_ms
  * FoldedPreconditioner::ApplyPreconditionerComplex (FoldedPreconditioner.cpp:578-629). */
 int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, double* y);
 
+/* Allocation-free form of mtcocr_apply, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. x and y are DEVICE pointers to one complex vector each (2*nb
+ * interleaved doubles, FP64) in the caller's row order, so the handle must hold no device order
+ * (mtcocr_set_mesh_order not applied: MTCOCR_ERR_STATE otherwise). omega: the frequency of a
+ * K/S matrix (A = K + i*omega*S); 1 for a block2x2 matrix. op as mtcocr_apply (0 SpMV,
+ * 1 preconditioner). The work goes to `stream` (a hipStream_t; NULL: the handle's stream) and
+ * is not waited for. It calls no allocating routine: the D_N^-1 diagonal, the node vector of
+ * the preconditioner and the SpMV's partial sums live in a 32 MiB scratch array that every
+ * GPU the library runs on holds for the life of the process, which limits the preconditioner to
+ * meshes of about one million interior nodes (MTCOCR_ERR_ARG beyond, with a message). All
+ * calls on one GPU share that array: calls that overlap on one GPU (other streams, other
+ * threads, other handles) race and must be serialised by the caller. The result equals
+ * mtcocr_apply's on the same handle without a batch, bit for bit. */
+int mtcocr_apply_noalloc(mtcocr_handle* h, int op, double omega, const double* x, double* y,
+                         void* stream);
+
 /* Matrix statistics of the device layout: complex / real non-zeros of the full matrix (explicit
  * zeros dropped), number of interior nodes m of G, device bytes held by the handle. */
 int mtcocr_matrix_info(mtcocr_handle* h, int64_t* nnz_complex, int64_t* nnz_real, int64_t* m,

@@ -101,6 +101,8 @@ SIGNATURES = {
     "mtcocr_bench_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mtcocr_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f64p, f64p]),
+    "mtcocr_apply_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
     "mtcocr_matrix_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mtcocr_sigma_gradient": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, f64p, f64p, f64p,

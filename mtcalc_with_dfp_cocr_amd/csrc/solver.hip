@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1470,6 +1471,21 @@ __global__ __launch_bounds__(kScalarThreads) void s_reduce_probe(const double* p
   if (threadIdx.x < P) out[threadIdx.x] = tot[threadIdx.x];
 }
 
+// state of one running system (SP = 1) for the allocation-free entry point: its scalars, the
+// running count and omega, written on the device (no host copy, so that the launch sequence
+// can be captured)
+__global__ void k_noalloc_init(SysScalars* sc, int* nrun, double* om, double omega) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    SysScalars z = SysScalars();
+    z.status = ST_RUNNING;
+    z.outer_status = ST_RUNNING;
+    z.rel = 1.0;
+    *sc = z;
+    *nrun = 1;
+    *om = omega;
+  }
+}
+
 // ---- mixed precision: FP64 outer refinement ------------------------------------------------
 
 // after the FP64 true residual: relative residual, convergence / stall, inner set-up
@@ -1702,9 +1718,42 @@ struct mtcocr_handle {
   }
 };
 
+// Device scratch of the allocation-free entry point (mtcocr_apply_noalloc): the per-system
+// scalars, the SpMV's partial sums and the preconditioner's node vectors live here instead of in
+// buffers the handle allocates. Each device holds one copy of the array for the life of the
+// process; host code uses the per-device address from hipGetSymbolAddress (scratch_ptr), never
+// the array's name, which in host code is only a stand-in.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local char g_err[512];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_rc[kMaxDevices];
+bool g_scratch_tried[kMaxDevices];
+
+// Address of this device's scratch array, looked up once per device with the device current
+// (the handle's set-up calls it); null when the lookup failed.
+void* scratch_ptr(int device, hipError_t* rc_out) {
+  if (device < 0 || device >= kMaxDevices) {
+    if (rc_out) *rc_out = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_tried[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_rc[device] = e;
+    g_scratch_ptr[device] = (e == hipSuccess) ? p : nullptr;
+    g_scratch_tried[device] = true;
+  }
+  if (rc_out) *rc_out = g_scratch_rc[device];
+  return g_scratch_ptr[device];
+}
 
 #define HIPC(expr)                                                                        \
   do {                                                                                    \
@@ -2553,6 +2602,9 @@ int mtcocr_create(int device, mtcocr_handle** out) {
   HIPC(hipEventCreate(&h->ev0));
   HIPC(hipEventCreate(&h->ev1));
   HIPC(hipHostMalloc((void**)&h->h_flags, 64, hipHostMallocDefault));
+  // the allocation-free entry point's scratch address on this device (a failed lookup only
+  // disables that entry point, which then reports it)
+  (void)scratch_ptr(device, nullptr);
   *out = h.release();
   return 0;
 }
@@ -3151,6 +3203,65 @@ int mtcocr_apply(mtcocr_handle* h, int op, int precision, const double* x, doubl
     y[2 * o] = out[(size_t)r * sp].x;
     y[2 * o + 1] = out[(size_t)r * sp].y;
   }
+  return 0;
+}
+
+int mtcocr_apply_noalloc(mtcocr_handle* h, int op, double omega, const double* x, double* y,
+                         void* stream) {
+  if (!h || !x || !y || (op != 0 && op != 1)) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || (op == 1 && (!h->have_g || h->g_nb != h->nb))) return MTCOCR_ERR_STATE;
+  if (!h->order.empty()) return MTCOCR_ERR_STATE;  // x and y are in the caller's row order
+  if (!h->ks_form && omega != 1.0) return MTCOCR_ERR_ARG;
+  if (!(omega > 0.0)) return MTCOCR_ERR_ARG;
+  HIPC(hipSetDevice(h->device));
+  hipError_t e = hipSuccess;
+  unsigned char* base = static_cast<unsigned char*>(scratch_ptr(h->device, &e));
+  if (!base) {
+    std::snprintf(g_err, sizeof(g_err), "scratch address lookup failed: %s", hipGetErrorString(e));
+    return MTCOCR_ERR_HIP;
+  }
+  // scratch layout (256-byte aligned parts): scalars | running count, omega | SpMV partials
+  // (one (re, im) pair per workgroup) | D_N^-1 | help (m complex values each)
+  const int grid_s = grid_for(h->nb, h->cap_spmv[0][0]);
+  const int grid_e = grid_for(h->nb, h->cap_vec[0][0]);
+  const int grid_n = grid_for(std::max(h->m, 1), h->cap_node[0][0]);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_sc = 0, o_run = al(sizeof(SysScalars)), o_part = o_run + 256;
+  const size_t o_dn = o_part + al(sizeof(double) * 2 * (size_t)grid_s);
+  const size_t o_help = o_dn + al(sizeof(double2) * (size_t)std::max(h->m, 1));
+  const size_t need = (op == 0) ? o_dn : o_help + al(sizeof(double2) * (size_t)std::max(h->m, 1));
+  if (need > kScratchBytes) {
+    std::snprintf(g_err, sizeof(g_err),
+                  "mtcocr_apply_noalloc needs %zu bytes of scratch, the limit is %zu (m = %d)",
+                  need, kScratchBytes, h->m);
+    return MTCOCR_ERR_ARG;
+  }
+  SysScalars* sc = reinterpret_cast<SysScalars*>(base + o_sc);
+  int* nrun = reinterpret_cast<int*>(base + o_run);
+  double* om = reinterpret_cast<double*>(base + o_run + 64);
+  double* part = reinterpret_cast<double*>(base + o_part);
+  double2* dn = reinterpret_cast<double2*>(base + o_dn);
+  double2* help = reinterpret_cast<double2*>(base + o_help);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  const double2* xv = reinterpret_cast<const double2*>(x);
+  double2* yv = reinterpret_cast<double2*>(y);
+  k_noalloc_init<<<1, 64, 0, st>>>(sc, nrun, om, omega);
+  if (op == 0) {
+    // the sliced-ELL SpMV at one system, as mtcocr_apply runs it without a batch
+    launch_spmv<double, 1, 0>(h, 0, grid_s, st, om, xv, yv, nullptr, nullptr, nullptr, sc, nrun,
+                              part);
+  } else {
+    const int g2 = std::min(4096, (int)(((long long)h->m + 255) / 256) + 1);
+    k_diag_n<double, 1><<<g2, 256, 0, st>>>(ell_view<double>(h), g_view<double>(h), om, dn);
+    const GView<double> G = g_view<double>(h);
+    k_pc_node<double, 1, false><<<grid_n, kBlock, 0, st>>>(G, xv, nullptr, dn, help, sc, nrun);
+    DiagA da;
+    da.a = h->da64.as<double>();
+    da.b = h->db64.as<double>();
+    da.omega = om;
+    k_pc_edge_plain<double, 1><<<grid_e, kBlock, 0, st>>>(G, xv, da, help, yv, sc, nrun);
+  }
+  HIPC(hipGetLastError());
   return 0;
 }
 
