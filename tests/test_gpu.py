@@ -273,6 +273,10 @@ def test_dropin_executable(O, M, c1_model, c1_system, c1_mesh, tmp_path):
 
 
 def test_dropin_batch_executable(O, M, c1_model, c1_mesh, tmp_path):
+    """--batch over three directories of one mesh: every v3.dat against the oracle; each kit
+    holds the system's share of the batch time (batch time / 3, printed by the process); a
+    directory whose off-diagonal blocks are not K + i omega S of the first, or whose kuslau
+    tolerance differs, is refused (exit 1)."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     say("drop-in executable --batch")
@@ -283,10 +287,125 @@ def test_dropin_batch_executable(O, M, c1_model, c1_mesh, tmp_path):
     r = subprocess.run([EXE, "--batch", *map(str, dirs)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
+    batch_s = float(r.stdout.split("batch time=")[1].split()[0])
     for d in dirs:
         s = M.read_block_system(d)
         xo, so, _ = O.pcocr(s, c1_mesh, eps=1e-8)
         assert rel(np.fromfile(d / "v3.dat", np.float64), xo) <= 1e-5
+        kit_s = float((d / "kit").read_text().split("time=")[1].split()[0])
+        assert kit_s == pytest.approx(batch_s / 3, abs=2e-6)
+    # one off-diagonal block of the last directory changed: not the first one's K + i omega S
+    gg = np.fromfile(dirs[2] / "gg", np.float64)
+    gg[len(gg) // 2] *= 1.5
+    gg.tofile(dirs[2] / "gg")
+    r = subprocess.run([EXE, "--batch", *map(str, dirs)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1 and "K + i*omega*S" in r.stderr
+    # another tolerance in kuslau
+    (dirs[1] / "kuslau").write_text(f"{2 * s.nb}\n1.000000e-07\n10000\n")
+    r = subprocess.run([EXE, "--batch", str(dirs[0]), str(dirs[1])], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1 and "eps/maxiter" in r.stderr
+
+
+def test_gradient_and_matrix_sizes_must_agree(M, c1_system, c1_mesh, tiny_model):
+    """Call order: a gradient set for one mesh and then a matrix of another size leave the
+    handle without a usable gradient (solve and preconditioner apply return MTCOCR_ERR_STATE
+    before any launch); a gradient whose edge count differs from the matrix is refused
+    (MTCOCR_ERR_ARG); setting the matching gradient again restores the solver."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("gradient / matrix size checks")
+    st = M.assemble(tiny_model, 1.0, 1)
+    mt = M.mesh_arrays(tiny_model)
+    assert st.nb != c1_system.nb
+    with Solver(0) as H:
+        H.set_matrix(c1_system)
+        H.set_gradient(c1_mesh)
+        H.set_matrix(st)
+        with pytest.raises(RuntimeError, match=r"\(-13\)"):
+            H.solve(st.pr, precision="fp64")
+        with pytest.raises(RuntimeError, match=r"\(-13\)"):
+            H.apply("precond", st.pr)
+        with pytest.raises(RuntimeError, match=r"\(-10\)"):
+            H.set_gradient(c1_mesh)
+        H.set_gradient(mt)
+        x, s1 = H.solve(st.pr, precision="fp64")
+        assert s1["req"] == 0
+
+
+def test_hanging_node_preconditioner(O, M):
+    """The folded preconditioner on a mesh with hanging nodes (a 2:1 refined face, T-matrix
+    rows of 2 and 4 nodes): rows of G longer than a conforming mesh's 6 entries reach the
+    device's D_N set-up and the node and edge kernels. Against the oracle's BuildGMatrix,
+    BuildDiagComplex and ApplyPreconditionerComplex on the same arrays, with a complex-
+    symmetric matrix that couples every two edges of a node."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    import ctypes as C
+
+    from test_gradient import hanging_mesh
+
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    say("hanging-node preconditioner")
+    m = M.random_model(4, 3, 3, seed=2, h=100.0, n_pad=1, n_air=2, name="g")
+    ma = M.mesh_arrays(m)
+    edges, nc, ig_t, jg_t, gg_t = hanging_mesh(M, ma)
+    ntot = len(ig_t) - 1
+    nb = len(edges)
+    bound = np.concatenate([ma.is_node_bound, np.zeros(ntot - nc, np.int32)]).astype(np.int32)
+    # lower-triangle blocks: every pair of edges that share a node (hanging nodes through
+    # their T rows), complex
+    rng = np.random.default_rng(8)
+    node_edges = {}
+    for e, (u, v) in enumerate(edges):
+        for w in (u, v):
+            ks = [w] if w < nc else list(jg_t[ig_t[w]:ig_t[w + 1]])
+            for k in ks:
+                node_edges.setdefault(int(k), set()).add(e)
+    lower = {i: set() for i in range(nb)}
+    for es in node_edges.values():
+        for a in es:
+            for b in es:
+                if b < a:
+                    lower[a].add(b)
+    ig = np.zeros(nb + 1, np.int32)
+    jg = []
+    for i in range(nb):
+        cols = sorted(lower[i])
+        jg += cols
+        ig[i + 1] = ig[i] + len(cols)
+    jg = np.array(jg, np.int32)
+    ijg = np.arange(0, 2 * len(jg) + 1, 2, dtype=np.int32)
+    gg = 0.1 * rng.standard_normal(2 * len(jg))
+    idi = np.arange(0, 2 * nb + 1, 2, dtype=np.int32)
+    di = np.stack([4.0 + rng.random(nb), rng.random(nb)], 1).ravel()
+    s = M.BlockSystem(nb, ig, jg, idi, ijg, di, gg, np.zeros(2 * nb))
+    x = rng.standard_normal(2 * nb)
+    # oracle: BuildGMatrix with the T-matrix, the diagonals, one application
+    L = O.lib()
+    p = O.OrcDfp()
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert L.orc_build_g(C.byref(p), int(ma.kpar), nb, nc, ma.nvkat.astype(np.int32),
+                         ma.nver.astype(np.int32), ma.sig_flags.astype(np.float64),
+                         edges.ravel().astype(np.int32), vp(ig_t), vp(jg_t), vp(gg_t), bound) == 0
+    assert L.orc_build_diag(C.byref(p), ig, jg, idi, ijg, di, gg) == 0
+    ref = np.empty(2 * nb)
+    L.orc_apply_precond(C.byref(p), x, ref, 1)
+    L.orc_dfp_free(C.byref(p))
+    with Solver(0) as H:
+        H.set_matrix(s)
+        rc = H._L.mtcocr_set_gradient_mesh(H._h, int(ma.kpar), nb, nc, ma.nvkat.astype(np.int32),
+                                           ma.nver.astype(np.int32),
+                                           ma.sig_flags.astype(np.float64),
+                                           edges.ravel().astype(np.int32), vp(ig_t), vp(jg_t),
+                                           vp(gg_t), bound)
+        assert rc == 0
+        got = H.apply("precond", x, "fp64")
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
 def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
