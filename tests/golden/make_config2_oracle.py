@@ -1,0 +1,78 @@
+"""Writes config2_oracle.json: the oracle (the FP64 CPU restatement of COCR_FP) run to its stop
+test on every system of the 100x100x60-cell benchmark model (seed 0, 8 frequencies from 1e-3 to
+1e2 Hz x 2 polarizations, eps = 1e-8), recording per system its iteration count, its MRS
+(stop-test) residual and the true relative residual ||b - A x|| / ||b|| of its returned x, and
+per frequency the impedance tensor at a 5x5 surface grid (200 m spacing, centred) computed from
+its solutions with the package's receiver operator. The GPU parity test at full size compares
+against these numbers (the oracle takes about 15 s per system on 16 cores, too long to rerun
+for all 16 systems inside a test).
+
+    python tests/golden/make_config2_oracle.py [threads]
+"""
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path[:0] = [str(ROOT), str(ROOT / "oracle")]
+
+import oracle as O  # noqa: E402
+from mtcalc_with_dfp_cocr_amd import model as M  # noqa: E402
+from mtcalc_with_dfp_cocr_amd.receivers import ReceiverOperator, impedance  # noqa: E402
+
+OUT = Path(__file__).resolve().parent / "config2_oracle.json"
+GRID = np.arange(-400.0, 401.0, 200.0)
+
+
+def config2():
+    m = M.random_model(100, 100, 60, seed=0, h=50.0, name="bench")
+    freqs = M.frequencies(1e-3, 1e2, 8)
+    return m, freqs, [(f, a) for f in freqs for a in (1, 0)]
+
+
+def receivers():
+    rx, ry = np.meshgrid(GRID, GRID, indexing="ij")
+    return np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1)
+
+
+def z_tensor(R, m, f, x_x, x_y):
+    Ex, Hx = R.fields(M.to_complex(x_x), f, 1)
+    Ey, Hy = R.fields(M.to_complex(x_y), f, 0)
+    return impedance(Ex, Hx, Ey, Hy)
+
+
+def main():
+    threads = int(sys.argv[1]) if len(sys.argv) > 1 else min(len(os.sched_getaffinity(0)), 16)
+    m, freqs, cases = config2()
+    ma = M.mesh_arrays(m)
+    R = ReceiverOperator(m, receivers())
+    out = {"model": "random_model(100, 100, 60, seed=0, h=50.0)", "eps": 1e-8,
+           "threads": threads, "receivers": receivers().tolist(), "systems": [], "Z": []}
+    xs = {}
+    for k, (f, a) in enumerate(cases):
+        s = M.assemble(m, f, a)
+        t0 = time.time()
+        x, st, _ = O.pcocr(s, ma, eps=1e-8, maxiter=20000, nthreads=threads)
+        A = M.sparse_matrix(s)
+        b = M.to_complex(s.pr)
+        tr = float(np.linalg.norm(b - A @ M.to_complex(x)) / np.linalg.norm(b))
+        rec = {"k": k, "f": float(f), "pol": "X" if a else "Y", "req": st["req"],
+               "iter": st["iter"], "mrs_rel": st["residual_rel"], "true_rel": tr,
+               "seconds": time.time() - t0}
+        out["systems"].append(rec)
+        print(json.dumps(rec), flush=True)
+        xs[k] = x
+        if a == 0:  # both polarizations of this frequency are in
+            Z = z_tensor(R, m, f, xs[k - 1], xs[k])
+            out["Z"].append({"f": float(f), "re": Z.real.tolist(), "im": Z.imag.tolist()})
+            del xs[k - 1], xs[k]
+    OUT.write_text(json.dumps(out, indent=1))
+    print(f"wrote {OUT}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
