@@ -58,6 +58,8 @@ extern "C" {
 
 #define MTCOCR_FLAG_FP64_FINISH 1 /* mixed: refinement stalled, finished by the FP64 solver */
 
+#define MTCOCR_ORDER_GRID (-1) /* mtcocr_set_mesh_order: the grid order of a box mesh */
+
 #define MTCOCR_PREC_FP64 0  /* plain FP64 COCR, the reference's iteration and stop test */
 #define MTCOCR_PREC_MIXED 1 /* complex FP32 inner COCR + FP64 iterative refinement */
 
@@ -114,7 +116,11 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
  * nodes swept along z, and of the interior nodes after them. The matrix, the gradient and every
  * vector are held in that order on the device; inputs and outputs keep the caller's order.
  * Must precede set_matrix_* / set_gradient_mesh (MTCOCR_ERR_STATE otherwise); block = 0 restores
- * the caller's order. Per-row arithmetic is unchanged (each row keeps its entry order), dot
+ * the caller's order. block = MTCOCR_ORDER_GRID instead holds the rows in the grid order of a
+ * mesh that is one full box of nx x ny x nz nodes: the x-edges, then the y- and the z-edges,
+ * each set in (z, y, x) lexicographic order of its lower node; a matrix set afterwards whose
+ * rows fit the edge-element stencil (33 couplings at fixed offsets) also gets the structured
+ * stencil SpMV layout (mtcocr_set_launch layout 3). Per-row arithmetic is unchanged (each row keeps its entry order), dot
  * products regroup. Returns MTCOCR_ERR_ARG, keeping the caller's order, for a mesh that is not
  * structured hexahedral. mtcocr_pcocr_2x2_folded and the COCR_FP executable apply it with
  * block 16. */
@@ -187,7 +193,10 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
  *   1 row pack, one row per group of lanes (batches of 3..64 systems);
  *   2 row blocks: the row pack over 16-row blocks whose x values are staged in LDS (batches of
  *     3..16 systems, when no block has more than 192 distinct columns); where it does not
- *     apply, the row pack runs.
+ *     apply, the row pack runs;
+ *   3 structured stencil: 42 coefficients per row at fixed stencil positions, each wave
+ *     sweeping a column of rows along z (batches of 3..16 systems, matrix set in the grid order
+ *     and fitting the stencil); where it does not apply, the row blocks run.
  * The row layouts need the batch vectors under 4 GiB (nb x width x 16 bytes in FP64); above
  * that the sliced ELL runs. set_launch applies to both precisions and every batch width;
  * get_launch reports those of `precision` at the current batch width, the layout as the one
@@ -200,6 +209,14 @@ int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
                       int* cap_node, int* spmv_layout);
+
+/* Sweep length of the stencil SpMV (layout 3): rows of one edge column that a wave task
+ * sweeps along z (at least 1; default 16; longer sweeps reload fewer levels, shorter ones keep
+ * the concurrently swept region of x within the L2 cache). set applies to every width and
+ * precision; get reports the value at the current batch width. No counterpart in the
+ * reference. */
+int mtcocr_set_stencil_sweep(mtcocr_handle* h, int kc);
+int mtcocr_get_stencil_sweep(mtcocr_handle* h, int precision, int* kc);
 
 /* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),

@@ -30,6 +30,7 @@ def _f64(a):
 
 
 MAX_BATCH = 64  # systems of one batch (the C side rejects more)
+ORDER_GRID = -1  # set_mesh_order / mesh_order: the grid order of a box mesh
 
 
 def _vec(a, n: int, what: str) -> np.ndarray:
@@ -83,7 +84,9 @@ class Solver:
     # ---- set-up -------------------------------------------------------------------------
     def set_mesh_order(self, ma: MeshArrays, block: int = 16) -> None:
         """Hold the matrix, gradient and vectors in the mesh's pencil order on the device
-        (inputs and outputs keep the caller's order). Call before set_matrix / set_gradient."""
+        (inputs and outputs keep the caller's order), or with block = ORDER_GRID in the grid
+        order of a box mesh, which also enables the stencil SpMV. Call before set_matrix /
+        set_gradient."""
         check(self._L.mtcocr_set_mesh_order(self._h, int(ma.kpar), int(ma.kuzlov),
                                             _i32(ma.nver), int(len(ma.edges) // 2),
                                             _i32(ma.edges), int(block)), "mtcocr_set_mesh_order")
@@ -173,8 +176,8 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    LAYOUTS = {"ell": 0, "rowpack": 1, "rowblock": 2}
-    LAYOUT_NAMES = ("ell", "rowpack", "rowblock")
+    LAYOUTS = {"ell": 0, "rowpack": 1, "rowblock": 2, "stencil": 3}
+    LAYOUT_NAMES = ("ell", "rowpack", "rowblock", "stencil")
 
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
                    spmv_layout: str | None = None) -> None:
@@ -191,6 +194,16 @@ class Solver:
         return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
                 "spmv_layout": self.LAYOUT_NAMES[d.value]}
 
+    def set_stencil_sweep(self, kc: int) -> None:
+        """Rows a wave task of the stencil SpMV sweeps along z (every width and precision)."""
+        check(self._L.mtcocr_set_stencil_sweep(self._h, int(kc)), "mtcocr_set_stencil_sweep")
+
+    def get_stencil_sweep(self, precision: str = "fp64") -> int:
+        kc = C.c_int()
+        check(self._L.mtcocr_get_stencil_sweep(self._h, PRECISION[precision], C.byref(kc)),
+              "mtcocr_get_stencil_sweep")
+        return kc.value
+
     def autotune(self, precision: str = "fp64", reps: int = 10) -> dict:
         """Time the tunable kernels of the current batch at each candidate shape and keep the
         fastest (needs a batch on the handle: set_rhs or a previous solve_batch)."""
@@ -198,8 +211,11 @@ class Solver:
         check(self._L.mtcocr_autotune(self._h, PRECISION[precision], int(reps), C.byref(a),
                                       C.byref(b), C.byref(c), C.byref(d), C.byref(ms)),
               "mtcocr_autotune")
-        return {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
-                "spmv_layout": self.LAYOUT_NAMES[d.value], "spmv_ms": ms.value}
+        out = {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
+               "spmv_layout": self.LAYOUT_NAMES[d.value], "spmv_ms": ms.value}
+        if out["spmv_layout"] == "stencil":
+            out["stencil_sweep"] = self.get_stencil_sweep(precision)
+        return out
 
     def apply(self, op: str, x, precision: str = "fp64"):
         """Kernel-level hook: op "spmv" (A x) or "precond" (folded preconditioner)."""

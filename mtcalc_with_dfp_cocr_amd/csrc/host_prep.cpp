@@ -2,6 +2,7 @@
 #include "host_prep.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -399,14 +400,18 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
   return 0;
 }
 
-int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
-                    int block, std::vector<int>& order) {
-  if (kpar <= 0 || n_nodes <= 0 || n_edges <= 0 || !nver || !edges || block < 1)
-    return MTCOCR_ERR_ARG;
-  // integer node coordinates from the cells' local vertex order (vertex l of a cell sits at
-  // offset (l&1, (l>>1)&1, (l>>2)&1) from vertex 0), propagated cell to cell
-  const int kUnset = std::numeric_limits<int>::min();
-  std::vector<int> cx(n_nodes, kUnset), cy(n_nodes, kUnset), cz(n_nodes, kUnset);
+namespace {
+
+constexpr int kUnset = std::numeric_limits<int>::min();
+
+// Integer node coordinates of a structured hexahedral mesh from the cells' local vertex order
+// (vertex l of a cell sits at offset (l&1, (l>>1)&1, (l>>2)&1) from vertex 0), propagated cell
+// to cell; nodes of no cell stay kUnset. Fails when the cells are not structured hexahedra.
+int mesh_coords(int kpar, int n_nodes, const int* nver, std::vector<int>& cx,
+                std::vector<int>& cy, std::vector<int>& cz) {
+  cx.assign(n_nodes, kUnset);
+  cy.assign(n_nodes, kUnset);
+  cz.assign(n_nodes, kUnset);
   std::vector<int64_t> nptr(n_nodes + 1, 0);
   for (int64_t c = 0; c < kpar; ++c)
     for (int l = 0; l < 8; ++l) {
@@ -458,6 +463,18 @@ int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const i
       }
     }
   }
+  return 0;
+}
+
+}  // namespace
+
+int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                    int block, std::vector<int>& order) {
+  if (kpar <= 0 || n_nodes <= 0 || n_edges <= 0 || !nver || !edges || block < 1)
+    return MTCOCR_ERR_ARG;
+  std::vector<int> cx, cy, cz;
+  const int rc = mesh_coords(kpar, n_nodes, nver, cx, cy, cz);
+  if (rc) return rc;
   // edge key: (pencil x, pencil y, z, y, x, direction) of the edge's lower end
   struct Key {
     int px, py, z, y, x, d;
@@ -486,13 +503,103 @@ int order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const i
   return 0;
 }
 
+int grid_order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                         std::vector<int>& order, GridDims& g) {
+  if (kpar <= 0 || n_nodes <= 0 || n_edges <= 0 || !nver || !edges) return MTCOCR_ERR_ARG;
+  std::vector<int> cx, cy, cz;
+  int rc = mesh_coords(kpar, n_nodes, nver, cx, cy, cz);
+  if (rc) return rc;
+  int lo[3] = {std::numeric_limits<int>::max(), std::numeric_limits<int>::max(),
+               std::numeric_limits<int>::max()};
+  int hi[3] = {kUnset, kUnset, kUnset};
+  for (int v = 0; v < n_nodes; ++v) {
+    if (cx[v] == kUnset) return MTCOCR_ERR_ARG;  // a node of no cell
+    const int c[3] = {cx[v], cy[v], cz[v]};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], c[a]);
+      hi[a] = std::max(hi[a], c[a]);
+    }
+  }
+  g.nx = hi[0] - lo[0] + 1;
+  g.ny = hi[1] - lo[1] + 1;
+  g.nz = hi[2] - lo[2] + 1;
+  if ((int64_t)g.nx * g.ny * g.nz != n_nodes) return MTCOCR_ERR_ARG;
+  if (g.count(0) + g.count(1) + g.count(2) != n_edges) return MTCOCR_ERR_ARG;
+  order.assign(n_edges, -1);
+  for (int e = 0; e < n_edges; ++e) {
+    const int a = edges[(int64_t)e * 2], b = edges[(int64_t)e * 2 + 1];
+    if (a < 0 || b < 0 || a >= n_nodes || b >= n_nodes) return MTCOCR_ERR_ARG;
+    const int dx = cx[b] - cx[a], dy = cy[b] - cy[a], dz = cz[b] - cz[a];
+    if (std::abs(dx) + std::abs(dy) + std::abs(dz) != 1) return MTCOCR_ERR_ARG;
+    const int d = dx ? 0 : (dy ? 1 : 2);
+    const int i = std::min(cx[a], cx[b]) - lo[0], j = std::min(cy[a], cy[b]) - lo[1],
+              k = std::min(cz[a], cz[b]) - lo[2];
+    const int64_t r = g.base(d) + ((int64_t)k * g.nyd(d) + j) * g.nxd(d) + i;
+    if (order[r] != -1) return MTCOCR_ERR_ARG;  // two edges on one slot
+    order[r] = e;
+  }
+  return 0;
+}
+
+int stencil_from_rows(const SlicedEll& E, const GridDims& g, std::vector<double>& coef) {
+  const int nb = E.nb;
+  if (nb != g.count(0) + g.count(1) + g.count(2) || E.rp.size() != (size_t)nb + 1)
+    return MTCOCR_ERR_ARG;
+  auto decode = [&](int r, int& d, int& i, int& j, int& k) {
+    d = r < g.base(1) ? 0 : (r < g.base(2) ? 1 : 2);
+    int64_t q = r - g.base(d);
+    i = int(q % g.nxd(d));
+    q /= g.nxd(d);
+    j = int(q % g.nyd(d));
+    k = int(q / g.nyd(d));
+  };
+  // slot of (column direction, di, dj, dk) within a type-d row, -1 outside the stencil
+  auto slot = [&](int d, int dt, int di, int dj, int dk) {
+    for (int c = 0; c < kStcCols[d]; ++c) {
+      const StencilCol& s = kStc[d][c];
+      if (s.dt == dt && s.di == di && s.dj == dj && dk >= s.lo && dk <= s.hi)
+        return stc_slot(d, c, dk);
+    }
+    return -1;
+  };
+  coef.assign((size_t)nb * kStcCoef, 0.0);
+  for (int r = 0; r < nb; ++r) {
+    int d, i, j, k;
+    decode(r, d, i, j, k);
+    double* row = coef.data() + (size_t)r * kStcCoef;
+    const int sd = slot(d, d, 0, 0, 0);
+    row[sd] = E.diag_a[r];
+    row[sd + 1] = E.diag_b[r];
+    const int ncx = E.rb[r + 1] - E.rb[r];
+    for (int q = E.rp[r]; q < E.rp[r + 1]; ++q) {
+      int d2, i2, j2, k2;
+      decode(E.pcol[q], d2, i2, j2, k2);
+      const int s = slot(d, d2, i2 - i, j2 - j, k2 - k);
+      if (s < 0) return MTCOCR_ERR_ARG;
+      const bool cplx = (q - E.rp[r]) < ncx;  // the row pack holds complex entries first
+      if (d2 == d) {
+        row[s] = E.pa[q];
+        row[s + 1] = cplx ? E.pb[E.rb[r] + (q - E.rp[r])] : 0.0;
+      } else {
+        if (cplx) return MTCOCR_ERR_ARG;
+        row[s] = E.pa[q];
+      }
+    }
+  }
+  return 0;
+}
+
 }  // namespace mtcocr
 
 extern "C" int mtcocr_mesh_order(int kpar, int n_nodes, const int* nver, int n_edges,
                                  const int* edges, int block, int* order) {
   if (!order) return MTCOCR_ERR_ARG;
   std::vector<int> ord;
-  const int rc = mtcocr::order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
+  mtcocr::GridDims dims;
+  const int rc =
+      block == MTCOCR_ORDER_GRID
+          ? mtcocr::grid_order_from_mesh(kpar, n_nodes, nver, n_edges, edges, ord, dims)
+          : mtcocr::order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
   if (rc) return rc;
   std::memcpy(order, ord.data(), sizeof(int) * (size_t)n_edges);
   return 0;

@@ -84,6 +84,70 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
                    const double* gg_t, const int* is_node_bound, const int* edge_order,
                    GradientEll& out);
 
+// Structured-mesh stencil of the lowest-order edge elements. On a box of nx x ny x nz nodes the
+// edges of direction d (0 x, 1 y, 2 z) at lower node (i, j, k) form boxes of
+// (nx - [d==0]) x (ny - [d==1]) x (nz - [d==2]); in the grid order the rows are those boxes one
+// after the other (x-edges, y-edges, z-edges), each in (k, j, i) lexicographic order. An edge
+// couples with the 33 edges of its 4 cells. They are grouped in "columns" (direction d', offsets
+// di, dj) holding the neighbours at dk = lo .. hi: for d = 0 and 1, 13 columns (3 parallel ones
+// with dk -1..1, 4 of one cross direction with dk -1..1, 6 of z with dk -1..0); for d = 2, 21
+// columns (9 parallel with dk 0, 6 + 6 cross with dk 0..1). A row's 42 coefficients are stored
+// column by column and dk ascending: the 9 parallel entries complex (a, b: a + i*omega*b), the
+// 24 cross entries real (a). The tables are shared by the host layout and the GPU kernel.
+struct StencilCol {
+  signed char dt, di, dj, lo, hi;
+};
+constexpr int kStcCoef = 42;              // doubles per row
+constexpr int kStcMaxCols = 21;
+constexpr int kStcCols[3] = {13, 13, 21};
+constexpr StencilCol kStc[3][kStcMaxCols] = {
+    {{0, 0, -1, -1, 1}, {0, 0, 0, -1, 1}, {0, 0, 1, -1, 1},                       // x (parallel)
+     {1, 0, -1, -1, 1}, {1, 0, 0, -1, 1}, {1, 1, -1, -1, 1}, {1, 1, 0, -1, 1},   // y
+     {2, 0, -1, -1, 0}, {2, 0, 0, -1, 0}, {2, 0, 1, -1, 0},                      // z
+     {2, 1, -1, -1, 0}, {2, 1, 0, -1, 0}, {2, 1, 1, -1, 0}},
+    {{1, -1, 0, -1, 1}, {1, 0, 0, -1, 1}, {1, 1, 0, -1, 1},                       // y (parallel)
+     {0, -1, 0, -1, 1}, {0, -1, 1, -1, 1}, {0, 0, 0, -1, 1}, {0, 0, 1, -1, 1},   // x
+     {2, -1, 0, -1, 0}, {2, 0, 0, -1, 0}, {2, 1, 0, -1, 0},                      // z
+     {2, -1, 1, -1, 0}, {2, 0, 1, -1, 0}, {2, 1, 1, -1, 0}},
+    {{2, -1, -1, 0, 0}, {2, 0, -1, 0, 0}, {2, 1, -1, 0, 0},                       // z (parallel)
+     {2, -1, 0, 0, 0}, {2, 0, 0, 0, 0}, {2, 1, 0, 0, 0},
+     {2, -1, 1, 0, 0}, {2, 0, 1, 0, 0}, {2, 1, 1, 0, 0},
+     {0, -1, -1, 0, 1}, {0, -1, 0, 0, 1}, {0, -1, 1, 0, 1},                       // x
+     {0, 0, -1, 0, 1}, {0, 0, 0, 0, 1}, {0, 0, 1, 0, 1},
+     {1, -1, -1, 0, 1}, {1, 0, -1, 0, 1}, {1, 1, -1, 0, 1},                       // y
+     {1, -1, 0, 0, 1}, {1, 0, 0, 0, 1}, {1, 1, 0, 0, 1}}};
+
+// offset (in doubles) of the coefficient of column c at dk within a type-d row
+constexpr int stc_slot(int d, int c, int dk) {
+  int o = 0;
+  for (int q = 0; q < c; ++q) {
+    const int w = kStc[d][q].hi - kStc[d][q].lo + 1;
+    o += w * (kStc[d][q].dt == d ? 2 : 1);
+  }
+  return o + (dk - kStc[d][c].lo) * (kStc[d][c].dt == d ? 2 : 1);
+}
+
+struct GridDims {
+  int nx = 0, ny = 0, nz = 0;  // nodes per axis
+  constexpr int nxd(int d) const { return nx - (d == 0); }
+  constexpr int nyd(int d) const { return ny - (d == 1); }
+  constexpr int nzd(int d) const { return nz - (d == 2); }
+  constexpr int64_t count(int d) const { return (int64_t)nxd(d) * nyd(d) * nzd(d); }
+  constexpr int64_t base(int d) const {
+    return d == 0 ? 0 : d == 1 ? count(0) : count(0) + count(1);
+  }
+};
+
+// The grid order (new -> old edge) of a structured hexahedral mesh that is one full box of
+// nodes (every edge of the box present once); fails with a negative code otherwise.
+int grid_order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
+                         std::vector<int>& order, GridDims& dims);
+
+// Stencil coefficients (nb x kStcCoef, row-major) of a matrix held in the grid order (its row
+// pack and diagonal in `E`). Fails (negative code) when a row has an entry outside its stencil
+// or a complex entry in a cross position; the layout then does not apply.
+int stencil_from_rows(const SlicedEll& E, const GridDims& dims, std::vector<double>& coef);
+
 // Locality order of the edges of a structured hexahedral mesh (cells as nver[kpar][14] with the
 // reference's vertex order, edges as node pairs): pencils of block x block nodes in x-y, each
 // swept along z, so that the rows a GPU works on together gather from a small, cache-resident

@@ -638,6 +638,18 @@ __device__ inline T2 buffer_gather(__amdgpu_buffer_rsrc_t r, unsigned off) {
   }
 }
 
+// the same with a wave-uniform byte offset (the instruction's scalar offset operand)
+template <class T2>
+__device__ inline T2 buffer_gather_so(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+  if constexpr (sizeof(T2) == 16) {
+    return __builtin_bit_cast(
+        T2, __builtin_amdgcn_raw_buffer_load_b128(r, int(off), int(soff), 0));
+  } else {
+    static_assert(sizeof(T2) == 8, "float2 or double2");
+    return __builtin_bit_cast(T2, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), int(soff), 0));
+  }
+}
+
 constexpr int kBufferDword3 = 0x00020000;  // raw buffer descriptor word 3 (CDNA)
 // A byte offset past every buffer built by buf_rsrc: raw buffer loads there return 0 and
 // touch no memory, which replaces the branch around a load that must not happen.
@@ -925,6 +937,282 @@ __global__ __launch_bounds__(kBlock) void k_spmv_blk(BlkView A, int hmax, const 
     }
     cur = nxt;
     __syncthreads();  // the next tile's staging overwrites xs
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// Structured-mesh stencil SpMV (batch widths 4..16) over the grid order of a box mesh (rows:
+// the x-, y- and z-edge boxes, each in (k, j, i) order; host_prep.h). Every row couples with a
+// fixed 33-point stencil whose positions follow from the row's (direction, i, j, k), so no
+// column indices are stored and no halo is staged: the matrix is 42 FP64 coefficients per row.
+// A wave owns RPW = 64/SP consecutive rows in i (SP lanes per row, one per system) of one
+// direction and one j, and sweeps a chunk of k. The stencil's points are grouped in columns
+// (direction, di, dj) spanning two or three k levels; each value x of column c at level L is
+// gathered once (one coalesced SP-value segment per row) and pushed into the rows L - dk it
+// couples with, so the sweep gathers 13 values per row and level (21 for z-edges) instead of 33.
+// Rows L-1..L+2 keep accumulators and their coefficients (spread over the row's SP lanes,
+// broadcast by DPP at use) in a ring of four; the next level's gathers and the coefficients two
+// rows ahead are issued before the current level's arithmetic. Neighbour positions are clamped
+// into their box: a row with a missing neighbour (a boundary row) has a zero coefficient there.
+// Same modes, outputs and partials as k_spmv.
+// Products with a coefficient broadcast from lane J of each 16-lane row, the broadcast folded
+// into the FMA (v_fmac_f64_dpp, row_newbcast). Written as inline asm because the compiler,
+// given separate DPP moves, schedules all of a level's broadcasts ahead of their FMAs and then
+// runs out of registers. The compiler does not count wait states for inline asm, so each block
+// starts with the two that a DPP read of a VGPR written by a VALU instruction needs (s_nop 1);
+// inside a block no instruction writes a DPP source.
+//   ar += c xr, ai += c xi
+template <int J>
+__device__ inline void dfma2_bc(double& ar, double& ai, double c, double xr, double xi) {
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %2, %3 row_newbcast:%5 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f64_dpp %1, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(ar), "+v"(ai)
+      : "v"(c), "v"(xr), "v"(xi), "i"(J));
+}
+//   (ar, ai) += (ca + i w cb)(xr + i xi), with nwxi = -w xi and wxr = w xr given
+template <int Ja, int Jb>
+__device__ inline void dfma4_bc(double& ar, double& ai, double ca, double cb, double xr, double xi,
+                                double nwxi, double wxr) {
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f64_dpp %1, %2, %5 row_newbcast:%8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f64_dpp %0, %3, %6 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f64_dpp %1, %3, %7 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(ar), "+v"(ai)
+      : "v"(ca), "v"(cb), "v"(xr), "v"(xi), "v"(nwxi), "v"(wxr), "i"(Ja), "i"(Jb));
+}
+
+struct StcView {
+  const double* coef;  // nb x kStcCoef, row-major
+  int nx, ny, nz;      // nodes per axis
+  int kc;              // rows of one sweep (k-chunk)
+  int nic;             // i-chunks of RPW rows
+  long long ntasks;    // 3 * nkc * ny * nic wave tasks
+  int nb;
+};
+
+// Output of one (row, system) of the stencil kernel, as spmv_store, through raw buffers: when
+// `ok` is false the offset is kOob, so the stores are dropped, the loads return 0 and the
+// partial sums receive nothing.
+template <class T, int MODE>
+__device__ inline void stc_store(bool ok, unsigned off, double yr, double yi,
+                                 __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t y_r,
+                                 __amdgpu_buffer_rsrc_t b_r, __amdgpu_buffer_rsrc_t ym_r,
+                                 __amdgpu_buffer_rsrc_t sm_r, double (&v)[2]) {
+  using T2 = typename Vec2<T>::type;
+  const unsigned o = ok ? off : kOob;
+  auto store = [&](__amdgpu_buffer_rsrc_t r, T2 val) {
+    if constexpr (sizeof(T2) == 16)
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, val), r, int(o), 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, val), r, int(o), 0, 0);
+  };
+  if (MODE == 0) {
+    T2 out;
+    out.x = T(yr);
+    out.y = T(yi);
+    store(y_r, out);
+    const T2 xt = buffer_gather<T2>(xr, o);  // 0 when the row is not stored
+    // the dot uses the stored (rounded) product, as the reference's ScalCmplx(z, s) does
+    v[0] += double(out.x) * double(xt.x) - double(out.y) * double(xt.y);
+    v[1] += double(out.x) * double(xt.y) + double(out.y) * double(xt.x);
+  } else {
+    const T2 bv = buffer_gather<T2>(b_r, o);
+    T2 out;
+    out.x = T(double(bv.x) - yr);
+    out.y = T(double(bv.y) - yi);
+    store(y_r, out);
+    v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
+    const double rr = double(out.x) * double(out.x) + double(out.y) * double(out.y);
+    v[1] += ok ? rr : 0.0;
+    const T2 xt = buffer_gather<T2>(xr, o);
+    store(ym_r, xt);
+    store(sm_r, out);
+  }
+}
+
+template <class T, int SP, int MODE, int D>
+__device__ inline void stc_sweep(const StcView& A, int ic, int j, int kc, int s, int g, double w,
+                                 bool run, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t cr,
+                                 __amdgpu_buffer_rsrc_t yr_, __amdgpu_buffer_rsrc_t br_,
+                                 __amdgpu_buffer_rsrc_t ymr, __amdgpu_buffer_rsrc_t smr,
+                                 double (&v)[2]) {
+  using T2 = typename Vec2<T>::type;
+  constexpr int RPW = 64 / SP;
+  constexpr int NC = kStcCols[D];
+  constexpr int NCR = (kStcCoef + SP - 1) / SP;  // coefficient registers per lane and row
+  constexpr int LO = D == 2 ? 0 : -1, HI = 1;    // level range of the stencil (dk)
+  constexpr unsigned kRow = SP * sizeof(T2);      // bytes of one row of x (all systems)
+  const GridDims G{A.nx, A.ny, A.nz};
+  const int nxd = G.nxd(D), nyd = G.nyd(D), nzd = G.nzd(D);
+  const int i = ic * RPW + g;
+  const int k0 = kc * A.kc, k1 = min(k0 + A.kc, nzd);
+  if (j >= nyd || k0 >= nzd || ic * RPW >= nxd) return;  // the whole task is empty
+  const bool row_ok = i < nxd;
+  const int ir = min(i, nxd - 1);
+  // per column: the lane's byte offset in x (its i clamped into the column's box), the
+  // uniform row stride of one level, and the uniform part of the row index
+  unsigned vo[NC];
+  long long ub[NC], us[NC];
+  int nzc[NC];
+  static_for<0, NC>([&](auto C) {
+    constexpr StencilCol c = kStc[D][C];
+    const int nxc = G.nxd(c.dt), nyc = G.nyd(c.dt);
+    const int ic_ = min(max(i + c.di, 0), nxc - 1);
+    const int jc = min(max(j + c.dj, 0), nyc - 1);
+    vo[C] = unsigned(ic_) * kRow + unsigned(s) * unsigned(sizeof(T2));
+    ub[C] = G.base(c.dt) + (long long)jc * nxc;
+    us[C] = (long long)nyc * nxc;
+    nzc[C] = G.nzd(c.dt);
+  });
+  // the value of column C at level L (the level clamped into the column's box)
+  auto gather1 = [&](auto Cc, int L) {
+    constexpr int C = decltype(Cc)::value;
+    const int Lc = min(max(L, 0), nzc[C] - 1);
+    const unsigned so = unsigned((ub[C] + (long long)Lc * us[C]) * kRow);
+    return buffer_gather_so<T2>(xr, vo[C], so);
+  };
+  // coefficients of row k: lane s of the row holds slots s, s + SP, ... (0 past the row)
+  auto load_coef = [&](double (&dst)[NCR], int k) {
+    const int kk = min(max(k, 0), nzd - 1);
+    const long long r = G.base(D) + ((long long)kk * nyd + j) * nxd + ir;
+    const unsigned rb = unsigned(r) * unsigned(kStcCoef * 8);
+    static_for<0, NCR>([&](auto Q) {
+      const int slot = Q * SP + s;
+      dst[Q] = buf_f64(cr, slot < kStcCoef ? rb + unsigned(slot) * 8u : kOob);
+    });
+  };
+  // Columns are processed in batches of NBC; the values of the next NBUF - 1 batches (across
+  // level boundaries) are in flight while a batch computes, and scheduling barriers keep the
+  // compiler from hoisting more loads than that (which would cost occupancy). Batch b of level
+  // L0 + n is batch g = n * NB + b of the sweep and lives in buffer g mod NBUF.
+  constexpr int NBC = 4;
+  constexpr int NBUF = 4;
+  constexpr int NB = (NC + NBC - 1) / NBC;  // batches per level
+  T2 vb[NBUF][NBC];
+  double cf[4][NCR];
+  double ar[4], ai[4];
+  const int L0 = k0 + LO, L1 = k1 - 1 + HI;  // levels the rows k0..k1-1 need
+  auto load_batch = [&](T2 (&dst)[NBC], auto Bc, int L) {
+    constexpr int B = decltype(Bc)::value;
+    static_for<0, NBC>([&](auto Jc) {
+      constexpr int C = B * NBC + decltype(Jc)::value;
+      if constexpr (C < NC) dst[decltype(Jc)::value] = gather1(std::integral_constant<int, C>{}, L);
+    });
+  };
+  // Rings indexed relative to the first level L0: row L0 + m sits in slot m mod 4, so every
+  // index below is a compile-time constant (NBUF divides 4, the level unroll).
+  static_for<0, NBUF - 1>([&](auto Gc) {
+    constexpr int G = decltype(Gc)::value;
+    if (L0 + G / NB <= L1) load_batch(vb[G % NBUF], std::integral_constant<int, G % NB>{}, L0 + G / NB);
+  });
+  load_coef(cf[3], L0 - 1);
+  load_coef(cf[0], L0);
+  load_coef(cf[1], L0 + 1);
+  ar[3] = ai[3] = ar[0] = ai[0] = 0.0;
+  // level L = L0 + n with P = n mod 4
+  auto level = [&](auto Pc, int L) {
+    constexpr int P = decltype(Pc)::value;
+    load_coef(cf[(P + 2) % 4], L + 2);   // row L+2, first used at level L+1
+    ar[(P + 1) % 4] = ai[(P + 1) % 4] = 0.0;  // row L+1, first contribution at level L
+    static_for<0, NB>([&](auto Bc) {
+      constexpr int B = decltype(Bc)::value;
+      constexpr int g = P * NB + B;                 // mod NBUF, the sweep's batch index
+      constexpr int cur = g % NBUF;
+      constexpr int ahead = B + NBUF - 1;           // batch to issue, counted from this level
+      __builtin_amdgcn_sched_barrier(0);
+      if (L + ahead / NB <= L1)
+        load_batch(vb[(g + NBUF - 1) % NBUF], std::integral_constant<int, ahead % NB>{},
+                   L + ahead / NB);
+      __builtin_amdgcn_sched_barrier(0);
+      static_for<0, NBC>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        constexpr int C = B * NBC + J;
+        if constexpr (C < NC) {
+          constexpr StencilCol c = kStc[D][C];
+          const double xr_ = double(vb[cur][J].x), xi_ = double(vb[cur][J].y);
+          // a parallel column's coefficients are complex: the omega-scaled parts once per value
+          const double nwxi = c.dt == D ? -w * xi_ : 0.0, wxr = c.dt == D ? w * xr_ : 0.0;
+          static_for<c.lo, c.hi + 1>([&](auto DK) {
+            constexpr int dk = decltype(DK)::value;
+            constexpr int R = ((P - dk) % 4 + 4) % 4;  // ring slot of row L - dk
+            constexpr int o = stc_slot(D, C, dk);
+            if constexpr (c.dt == D)
+              dfma4_bc<o % SP, (o + 1) % SP>(ar[R], ai[R], cf[R][o / SP], cf[R][(o + 1) / SP],
+                                             xr_, xi_, nwxi, wxr);
+            else
+              dfma2_bc<o % SP>(ar[R], ai[R], cf[R][o / SP], xr_, xi_);
+          });
+        }
+      });
+    });
+    // row L - HI has all its levels: store it (without a branch, so that the compiler does
+    // not sink the row's last products into a conditional block away from their broadcasts)
+    const int k = L - HI;
+    constexpr int R = ((P - HI) % 4 + 4) % 4;
+    const bool ok = k >= k0 && k < k1 && row_ok && (run || MODE == 0);
+    const long long t = (G.base(D) + ((long long)min(max(k, 0), nzd - 1) * nyd + j) * nxd + ir) *
+                            SP + s;
+    stc_store<T, MODE>(ok, unsigned(t) * unsigned(sizeof(T2)), ar[R], ai[R], xr, yr_, br_, ymr,
+                       smr, v);
+  };
+  for (int L = L0; L <= L1; L += 4) {
+    level(std::integral_constant<int, 0>{}, L);
+    if (L + 1 > L1) break;
+    level(std::integral_constant<int, 1>{}, L + 1);
+    if (L + 2 > L1) break;
+    level(std::integral_constant<int, 2>{}, L + 2);
+    if (L + 3 > L1) break;
+    level(std::integral_constant<int, 3>{}, L + 3);
+  }
+}
+
+template <class T, int SP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_stc(StcView A, const double* omega,
+                                                     const typename Vec2<T>::type* __restrict__ x,
+                                                     typename Vec2<T>::type* __restrict__ y,
+                                                     const typename Vec2<T>::type* __restrict__ b,
+                                                     typename Vec2<T>::type* __restrict__ ym,
+                                                     typename Vec2<T>::type* __restrict__ sm,
+                                                     const SysScalars* __restrict__ sc,
+                                                     const int* __restrict__ nrun,
+                                                     double* partials) {
+  static_assert(SP == 16, "16-lane rows (the DPP row broadcast)");
+  using T2 = typename Vec2<T>::type;
+  if (*nrun == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int s = lane % SP, g = lane / SP;
+  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
+  const double w = omega[s];
+  double v[2] = {0.0, 0.0};
+  const long long vbytes = (long long)A.nb * SP * (long long)sizeof(T2);
+  const __amdgpu_buffer_rsrc_t xr = buf_rsrc(x, vbytes);
+  const __amdgpu_buffer_rsrc_t cr = buf_rsrc(A.coef, (long long)A.nb * kStcCoef * 8);
+  // output buffers; an absent one (null: b, ym, sm in MODE 0, ym and sm optional in MODE 1)
+  // gets zero records, so that its stores are dropped and its loads return 0
+  const __amdgpu_buffer_rsrc_t y_r = buf_rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t b_r = buf_rsrc(b, b ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t ym_r = buf_rsrc(ym, ym ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t sm_r = buf_rsrc(sm, sm ? vbytes : 0);
+  const long long ntiles = (A.ntasks + 3) / 4;
+  TileWalk tw(ntiles);
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long task = tile * 4 + wv;
+    if (task >= A.ntasks) continue;
+    const int d = int(task % 3);
+    long long q = task / 3;
+    const int ic = int(q % A.nic);
+    q /= A.nic;
+    const int j = int(q % A.ny);
+    const int kc = int(q / A.ny);
+    if (d == 0)
+      stc_sweep<T, SP, MODE, 0>(A, ic, j, kc, s, g, w, run, xr, cr, y_r, b_r, ym_r, sm_r, v);
+    else if (d == 1)
+      stc_sweep<T, SP, MODE, 1>(A, ic, j, kc, s, g, w, run, xr, cr, y_r, b_r, ym_r, sm_r, v);
+    else
+      stc_sweep<T, SP, MODE, 2>(A, ic, j, kc, s, g, w, run, xr, cr, y_r, b_r, ym_r, sm_r, v);
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -1643,6 +1931,12 @@ struct mtcocr_handle {
 
   // device row order (new -> reference edge index); empty = the reference's order
   std::vector<int> order;
+  // the grid order of a box mesh (mtcocr_set_mesh_order with MTCOCR_ORDER_GRID): its node
+  // counts, and the stencil coefficients of the matrix when every row fits the stencil
+  bool grid_order = false;
+  GridDims grid;
+  DevBuf stc;
+  bool stc_ok = false;
 
   // matrix (shared index arrays, FP64 and FP32 values)
   bool have_matrix = false;
@@ -1681,8 +1975,10 @@ struct mtcocr_handle {
   int cap_upd[2][7];
   // SpMV layout per precision and batch width: 0 sliced ELL (k_spmv), 1 row pack (k_spmv_rg,
   // batch widths 4..64), 2 halo-staged row blocks (k_spmv_blk, batch widths 4..16; where it
-  // does not apply, the row pack runs)
+  // does not apply, the row pack runs), 3 structured stencil (k_spmv_stc, batch widths 4..16 on
+  // a matrix held in the grid order; else the row blocks)
   int spmv_layout[2][7];
+  int stc_kc[2][7];  // the stencil kernel's sweep length (rows of k per wave task)
   static int sp_index(int sp) {
     int k = 0;
     while ((1 << k) < sp && k < 6) ++k;
@@ -1703,11 +1999,13 @@ struct mtcocr_handle {
     set_caps(8192, 65536, 2048, 2);
     for (auto& row : cap_upd)
       for (int& c : row) c = kMaxBlocks;  // the update pass: one tile per workgroup
+    for (auto& row : stc_kc)
+      for (int& c : row) c = 16;
   }
 
   int64_t device_bytes() const {
     const DevBuf* all[] = {&offc, &nc, &offr, &nr, &colc, &colr, &ac64, &bc64, &ar64, &da64,
-                           &db64, &rp, &rb, &pcol, &pa, &pb, &hoff, &halo, &lcol, &ldiag, &gcol, &gtcol, &gval64, &gtval64,
+                           &db64, &rp, &rb, &pcol, &pa, &pb, &hoff, &halo, &lcol, &ldiag, &stc, &gcol, &gtcol, &gval64, &gtval64,
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2, &cvr, &cvp, &cvs, &cvz, &cva, &cvw, &cvym,
                            &cvsm, &cvx, &cvx2, &cmap,
@@ -1851,6 +2149,28 @@ int blk_lds_bytes(const mtcocr_handle* h) {
   return kRowBlock * h->halo_max * int(sizeof(typename Vec2<TV>::type));
 }
 
+// view of the stencil layout for batch width SP (RPW = 64 / SP rows of a wave) and sweep kc
+template <int SP>
+StcView stc_view(const mtcocr_handle* h, int kc) {
+  StcView v;
+  v.coef = h->stc.as<double>();
+  v.nx = h->grid.nx;
+  v.ny = h->grid.ny;
+  v.nz = h->grid.nz;
+  v.kc = std::max(1, kc);
+  constexpr int RPW = 64 / SP;
+  v.nic = (h->grid.nx + RPW - 1) / RPW;
+  const long long nkc = (h->grid.nz + v.kc - 1) / v.kc;
+  v.ntasks = 3LL * nkc * h->grid.ny * v.nic;
+  v.nb = h->nb;
+  return v;
+}
+
+template <class TV, int SP>
+int stc_kc_of(const mtcocr_handle* h) {
+  return h->stc_kc[std::is_same<TV, float>::value ? 1 : 0][mtcocr_handle::sp_index(SP)];
+}
+
 // The row kernels address x with 32-bit byte offsets: batch vectors of nb x SP values of
 // `elem` bytes must stay under 4 GiB (else the sliced ELL runs)
 inline bool rows_fit(const mtcocr_handle* h, int sp, int elem) {
@@ -1867,6 +2187,14 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  double* part) {
   if (!rows_fit(h, SP, int(sizeof(typename Vec2<TV>::type)))) layout = 0;
   if constexpr (SP >= 4 && SP <= 16) {
+    if constexpr (SP == 16) {
+      if (layout == 3 && h->stc_ok) {
+        k_spmv_stc<TV, SP, MODE><<<grid, kBlock, 0, st>>>(stc_view<SP>(h, stc_kc_of<TV, SP>(h)),
+                                                          omega, x, y, b, ym, sm, sc, nrun, part);
+        return;
+      }
+    }
+    if (layout == 3) layout = 2;  // the stencil layout serves 16-lane rows only
     const int lds = blk_lds_bytes<TV>(h);
     if (layout == 2 && lds > 0) {
       k_spmv_blk<TV, SP, MODE><<<grid, kBlock, lds, st>>>(blk_view(h), h->halo_max, omega, x, y,
@@ -2557,6 +2885,16 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
     return rc;
   h->halo_max = E.hoff.empty() ? 0 : E.halo_max;
   h->nblk = E.hoff.empty() ? 0 : int(E.hoff.size()) - 1;
+  h->stc_ok = false;
+  h->stc.release();
+  if (h->grid_order) {
+    std::vector<double> coef;
+    if (stencil_from_rows(E, h->grid, coef) == 0 &&
+        (long long)E.nb * kStcCoef * 8 <= (long long)kOob) {
+      if ((rc = upload(h, h->stc, coef, st))) return rc;
+      h->stc_ok = true;
+    }
+  }
   HIPC(hipStreamSynchronize(st));
   h->have_matrix = true;
   // a gradient built for another edge count does not fit this matrix: set it again
@@ -2687,19 +3025,27 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
 
 int mtcocr_set_mesh_order(mtcocr_handle* h, int kpar, int n_nodes, const int* nver,
                           int n_edges, const int* edges, int block) {
-  if (!h || block < 0) return MTCOCR_ERR_ARG;
+  if (!h || (block < 0 && block != MTCOCR_ORDER_GRID)) return MTCOCR_ERR_ARG;
   if (h->have_matrix || h->have_g) return MTCOCR_ERR_STATE;  // the layouts are built in order
+  h->grid_order = false;
   if (block == 0) {
     h->order.clear();
     return 0;
   }
   std::vector<int> ord;
-  const int rc = order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
+  GridDims dims;
+  const int rc = block == MTCOCR_ORDER_GRID
+                     ? grid_order_from_mesh(kpar, n_nodes, nver, n_edges, edges, ord, dims)
+                     : order_from_mesh(kpar, n_nodes, nver, n_edges, edges, block, ord);
   if (rc) {
     h->order.clear();
     return rc;
   }
   h->order.swap(ord);
+  if (block == MTCOCR_ORDER_GRID) {
+    h->grid_order = true;
+    h->grid = dims;
+  }
   return 0;
 }
 
@@ -2938,7 +3284,7 @@ int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
   if (!h) return MTCOCR_ERR_ARG;
   auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
   if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
-  if (spmv_layout < -1 || spmv_layout > 2) return MTCOCR_ERR_ARG;
+  if (spmv_layout < -1 || spmv_layout > 3) return MTCOCR_ERR_ARG;
   h->set_caps(cap_spmv, cap_vec, cap_node, spmv_layout);
   drop_graph(h);  // the captured iterations hold the old grids
   return 0;
@@ -2951,6 +3297,7 @@ namespace {
 // the row pack where they do not apply)
 int effective_layout(const mtcocr_handle* h, int q, int k) {
   int l = h->spmv_layout[q][k];
+  if (l == 3 && !(h->stc_ok && k == 4)) l = 2;
   if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
   if (!rows_fit(h, 1 << k, q == 0 ? 16 : 8)) l = 0;
   return l;
@@ -2958,6 +3305,21 @@ int effective_layout(const mtcocr_handle* h, int q, int k) {
 }  // namespace
 
 extern "C" {
+
+int mtcocr_set_stencil_sweep(mtcocr_handle* h, int kc) {
+  if (!h || kc < 1) return MTCOCR_ERR_ARG;
+  for (auto& row : h->stc_kc)
+    for (int& c : row) c = kc;
+  drop_graph(h);  // captured launches hold the old task count
+  return 0;
+}
+
+int mtcocr_get_stencil_sweep(mtcocr_handle* h, int precision, int* kc) {
+  if (!h || !kc) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  *kc = h->stc_kc[precision][mtcocr_handle::sp_index(h->nsys > 0 ? h->sp : 1)];
+  return 0;
+}
 
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
                       int* cap_node, int* spmv_layout) {
@@ -3027,16 +3389,33 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         HIPC(hipGetLastError());
       }
       for (int fam = 0; fam < 4; ++fam) {
-        const int nlayouts =
-            (fam != 0 || SP < 4 || !rows_fit(h, SP, int(sizeof(T2)))) ? 1
-            : (SP <= 16 && blk_lds_bytes<T>(h) > 0)                  ? 3
-                                                                      : 2;
-        for (int layout = 0; layout < nlayouts; ++layout) {
+        // SpMV layouts that apply to this batch (the others fall back to one of these)
+        std::vector<int> layouts{0};
+        if (fam == 0 && SP >= 4 && rows_fit(h, SP, int(sizeof(T2)))) {
+          layouts.push_back(1);
+          if (SP <= 16 && blk_lds_bytes<T>(h) > 0) layouts.push_back(2);
+          if (SP == 16 && h->stc_ok) layouts.push_back(3);
+        }
+        int best_kc = h->stc_kc[q][ki];
+        for (int layout : layouts) {
+          // the stencil kernel's sweep length: a few lengths, each at every cap
+          std::vector<int> kcs{0};
+          if (layout == 3) {
+            kcs = {2, 4, 8, 16, h->grid.nz};
+            std::sort(kcs.begin(), kcs.end());
+            kcs.erase(std::unique(kcs.begin(), kcs.end()), kcs.end());
+          }
+          for (int kc : kcs) {
+          if (layout == 3) h->stc_kc[q][ki] = kc;
           // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
           // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
           // twice that
           int per_cu = 0;
-          if (fam == 0 && layout == 2) {
+          if (fam == 0 && layout == 3) {
+            if constexpr (SP == 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_stc<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0 && layout == 2) {
             if constexpr (SP >= 4 && SP <= 16)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_blk<T, SP, 0>), kBlock,
@@ -3119,11 +3498,16 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
               best_ms[fam] = avg;
               best[fam] = cap;
               if (fam == 0) best_layout = layout;
+              if (fam == 0 && layout == 3) best_kc = kc;
             }
+          }
           }
         }
         *caps[fam] = best[fam];
-        if (fam == 0) h->spmv_layout[q][ki] = best_layout;
+        if (fam == 0) {
+          h->spmv_layout[q][ki] = best_layout;
+          h->stc_kc[q][ki] = best_kc;
+        }
       }
       return 0;
     };

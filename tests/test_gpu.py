@@ -96,6 +96,80 @@ def test_spmv_parity_batch_widths(O, M, c1_system, c1_mesh, nsys, layout):
                 assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nsys", [9, 16])
+def test_spmv_stencil_layout(O, M, c1_model, c1_mesh, nsys):
+    """The structured stencil SpMV (matrix and vectors in the grid order of the box mesh): every
+    system of a batch of 9 or 16 (both run 16-lane rows) against the oracle's product of that
+    system's matrix, FP64 and FP32 vectors; narrower batches fall back to the row blocks."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    say(f"stencil SpMV, {nsys} systems")
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)][:nsys]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    x = np.random.default_rng(17).standard_normal(2 * p.nb)
+    import scipy.sparse as sp
+
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+    full = lambda d, g: (sp.csr_matrix((g, (rows, p.jg)), shape=(p.nb, p.nb))
+                         + sp.csr_matrix((g, (p.jg, rows)), shape=(p.nb, p.nb)) + sp.diags(d))
+    A0 = full(kd, kg) + 1j * om[0] * full(sd, sg)  # system 0 of the batch: K + i omega_0 S
+    ref = M.to_interleaved(A0 @ M.to_complex(x))
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, ORDER_GRID)
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        B.set_rhs(prs, om)
+        B.set_launch(spmv_layout="stencil")
+        for prec, tol in (("fp64", 1e-13), ("mixed", 2e-6)):
+            assert B.get_launch(prec)["spmv_layout"] == "stencil"
+            got = B.apply("spmv", x, prec)  # system 0 of the batch
+            assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+        # 8 systems: the row blocks where their halos fit (not in the grid order here), else
+        # the row pack
+        B.set_rhs(prs[:8], om[:8])
+        assert B.get_launch("fp64")["spmv_layout"] in ("rowblock", "rowpack")
+
+
+def test_stencil_batched_solve(O, M, c1_model, c1_mesh):
+    """A batch of 16 systems (8 frequencies x 2 polarizations) solved with the stencil SpMV in the
+    grid order: every system against the oracle (iterations, x), and against the same batch with
+    the row-block SpMV in the pencil order (iterations within 2% + 2)."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    say("stencil batched solve, 16 systems")
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    runs = {}
+    for order, layout in ((ORDER_GRID, "stencil"), (16, "rowblock")):
+        with Solver(0) as B:
+            B.set_mesh_order(c1_mesh, order)
+            B.set_matrix_ks(p, kd, kg, sd, sg)
+            B.set_gradient(c1_mesh)
+            B.set_launch(spmv_layout=layout)
+            runs[layout] = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+    xs, ss = runs["stencil"]
+    xr, sr = runs["rowblock"]
+    for k in (0, 5, 10, 15):
+        f, a = cases[k]
+        xo, so, _ = O.pcocr(M.assemble(c1_model, f, a), c1_mesh, eps=1e-8)
+        say(f"case {f:.3g} Hz alpha={a}: stencil {ss[k]['iter']} it, oracle {so['iter']} it")
+        assert ss[k]["req"] == 0
+        assert abs(ss[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(xs[k], xo) <= 1e-5
+    for k in range(len(cases)):
+        assert ss[k]["req"] == 0 and abs(ss[k]["iter"] - sr[k]["iter"]) <= 2 + 0.02 * sr[k]["iter"]
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
 def test_preconditioner_parity(S, O, M, c1_system, c1_mesh, prec, tol):
     say(f"preconditioner {prec}")
@@ -696,22 +770,32 @@ def test_launcher_end_to_end(M, c1_model, tmp_path):
         assert d.max() <= 1e-6
 
 
-def test_full_size_residual(M):
-    """The benchmark's mesh at its full size (100x100x60 cells, 1.84 M edges): the lowest and
-    the highest frequency as one FP64 batch, the highest also in mixed precision. The true
-    relative residual ||b - A x|| / ||b||, from an FP64 host product with A = K + i omega S,
-    meets the reference's stop level (<= 3e-8: the reference's own x reaches 1-3e-8 at
-    eps = 1e-8, whose test is on the smoothed residual)."""
+def test_full_size_parity(M):
+    """The benchmark's mesh at its full size (100x100x60 cells, 1.84 M edges), all 16 systems
+    (8 frequencies x 2 polarizations) as one FP64 batch, against the oracle's own run of every
+    system to its stop test (tests/golden/config2_oracle.json):
+      * iterations within 2% + 2 of the oracle's;
+      * the true relative residual ||b - A x|| / ||b|| (FP64 host product, A = K + i omega S)
+        at most max(3e-8, 6x the oracle's): the stop test is on the MRS-smoothed residual, and
+        the raw iterate's residual at the stop point is rounding-path dependent (on system 13
+        the oracle's own x reaches 2.0e-8 on 8 threads and 1.06e-7 on 16: a factor 5.3);
+      * Z at the 5x5 grid within 1e-6 of the oracle's Z (Zxx and Zyy relative to |Zxy|).
+    The highest frequency is also solved in mixed precision, held to the same residual bound."""
     if not gpu_available():
         pytest.skip("no GPU visible")
+    import json
+    from pathlib import Path
+
     import scipy.sparse as sp
 
     from mtcalc_with_dfp_cocr_amd.cocr import Solver
+    from mtcalc_with_dfp_cocr_amd.receivers import ReceiverOperator, impedance
 
-    say("full size: config-2 mesh, 2 systems")
+    gold = json.loads((Path(__file__).parent / "golden" / "config2_oracle.json").read_text())
+    say("full size: config-2 mesh, 16 systems against the oracle's fixture")
     m = M.random_model(100, 100, 60, seed=0, h=50.0, name="bench")
     freqs = M.frequencies(1e-3, 1e2, 8)
-    cases = [(freqs[0], 1), (freqs[-1], 1)]
+    cases = [(f, a) for f in freqs for a in (1, 0)]
     p, kd, kg, sd, sg = M.assemble_ks(m)
     ma = M.mesh_arrays(m)
     prs = np.stack([M.assemble_rhs(m, f, a) for f, a in cases])
@@ -721,7 +805,7 @@ def test_full_size_residual(M):
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(ma)
         x, st = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
-        xm, sm = B.solve_batch(om[1:], prs[1:], eps=1e-8, precision="mixed")
+        xm, sm = B.solve_batch(om[-1:], prs[-1:], eps=1e-8, precision="mixed")
     rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
 
     def sym(di, gg):
@@ -732,16 +816,34 @@ def test_full_size_residual(M):
 
     def true_rel(k, xv):
         b = M.to_complex(prs[k])
-        return np.linalg.norm(b - (K @ M.to_complex(xv) + 1j * om[k] * (S @ M.to_complex(xv)))) \
-            / np.linalg.norm(b)
+        xc = M.to_complex(xv)
+        return np.linalg.norm(b - (K @ xc + 1j * om[k] * (S @ xc))) / np.linalg.norm(b)
 
-    for k in range(2):
+    for k, g in enumerate(gold["systems"]):
+        assert abs(g["f"] - cases[k][0]) <= 1e-12 * g["f"] and g["pol"] == "XY"[1 - cases[k][1]]
         r = true_rel(k, x[k])
-        say(f"f={cases[k][0]:.3g} Hz: {st[k]['iter']} it, true rel res {r:.2e}")
-        assert st[k]["req"] == 0 and r <= 3e-8
-    r = true_rel(1, xm[0])
-    say(f"mixed f={cases[1][0]:.3g} Hz: true rel res {r:.2e}")
-    assert sm[0]["req"] == 0 and r <= 3e-8
+        bound = max(3e-8, 6.0 * g["true_rel"])
+        say(f"{k}: f={g['f']:.3g} {g['pol']}: {st[k]['iter']} it (oracle {g['iter']}), "
+            f"true rel res {r:.2e} (oracle {g['true_rel']:.2e}, bound {bound:.2e})")
+        assert st[k]["req"] == 0 and st[k]["residual_rel"] < 1e-8
+        assert abs(st[k]["iter"] - g["iter"]) <= 2 + 0.02 * g["iter"]
+        assert r <= bound
+    R = ReceiverOperator(m, np.array(gold["receivers"]))
+    for zg in gold["Z"]:
+        f = zg["f"]
+        kx = [k for k, c in enumerate(cases) if c == (f, 1)][0]
+        Ex, Hx = R.fields(M.to_complex(x[kx]), f, 1)
+        Ey, Hy = R.fields(M.to_complex(x[kx + 1]), f, 0)
+        Z = impedance(Ex, Hx, Ey, Hy)
+        Zo = np.array(zg["re"]) + 1j * np.array(zg["im"])
+        scale = np.abs(Zo[:, 0, 1])
+        err = np.abs(Z - Zo) / scale[:, None, None]
+        err[:, 1, 0] = np.abs(Z[:, 1, 0] - Zo[:, 1, 0]) / np.abs(Zo[:, 1, 0])
+        say(f"Z at {f:.3g} Hz: max rel diff {err.max():.2e}")
+        assert err.max() <= 1e-6
+    r = true_rel(len(cases) - 1, xm[0])
+    say(f"mixed f={cases[-1][0]:.3g} Hz: true rel res {r:.2e}")
+    assert sm[0]["req"] == 0 and r <= max(3e-8, 6.0 * gold["systems"][-1]["true_rel"])
 
 
 def test_batch_with_zero_rhs_and_padding(O, M, c1_model, c1_mesh):

@@ -49,3 +49,32 @@ def test_not_hexahedral_refused(mesh):
     rc = lib().mtcocr_mesh_order(int(ma.kpar), int(ma.kuzlov), nver.astype(np.int32),
                                  len(order), ma.edges.astype(np.int32), 16, order)
     assert rc == -10
+
+
+def test_grid_order(mesh):
+    """The grid order of a box mesh: the x-edges, then the y- and the z-edges, each set in
+    (z, y, x) lexicographic order of its lower node, a permutation of the edges."""
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, mesh_order
+
+    ma, m = mesh
+    o = mesh_order(ma, block=ORDER_GRID)
+    n = len(ma.edges) // 2
+    assert np.array_equal(np.sort(o), np.arange(n))
+    nxn, nyn, nzn = len(m.x), len(m.y), len(m.z)
+    e = ma.edges.reshape(-1, 2)[o]
+    lo = np.minimum(e[:, 0], e[:, 1])
+    x, y, z = lo % nxn, (lo // nxn) % nyn, lo // (nxn * nyn)
+    d = np.where(np.abs(e[:, 1] - e[:, 0]) == 1, 0, np.where(np.abs(e[:, 1] - e[:, 0]) == nxn, 1, 2))
+    counts = [(nxn - 1) * nyn * nzn, nxn * (nyn - 1) * nzn, nxn * nyn * (nzn - 1)]
+    assert [int((d == k).sum()) for k in range(3)] == counts
+    key = np.stack([d, z, y, x], axis=1)
+    diff = np.diff(key, axis=0)
+    first_nonzero = np.argmax(diff != 0, axis=1)
+    assert np.all(diff[np.arange(len(diff)), first_nonzero] > 0)
+    # a mesh that is not one full box (an edge dropped) is refused
+    from mtcalc_with_dfp_cocr_amd._lib import lib
+
+    order = np.empty(n - 1, np.int32)
+    rc = lib().mtcocr_mesh_order(int(ma.kpar), int(ma.kuzlov), ma.nver.astype(np.int32), n - 1,
+                                 ma.edges[:-2].astype(np.int32), ORDER_GRID, order)
+    assert rc == -10
