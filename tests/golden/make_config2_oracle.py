@@ -49,7 +49,7 @@ def main():
     threads = int(sys.argv[1]) if len(sys.argv) > 1 else min(len(os.sched_getaffinity(0)), 16)
     m, freqs, cases = config2()
     ma = M.mesh_arrays(m)
-    R = ReceiverOperator(m, receivers())
+    R = ReceiverOperator(m, receivers(), smoothing=None)
     out = {"model": "random_model(100, 100, 60, seed=0, h=50.0)", "eps": 1e-8,
            "threads": threads, "receivers": receivers().tolist(), "systems": [], "Z": []}
     xs = {}

@@ -744,6 +744,20 @@ def test_impedance_parity_with_oracle(O, M, c1_model, c1_mesh):
     for i, j in off:
         np.testing.assert_allclose(rg[:, i, j], ro[:, i, j], rtol=2e-6)
         np.testing.assert_allclose(pg[:, i, j], po[:, i, j], atol=1e-4)
+    # the tipper from the same solutions; the block is centred under the receiver grid, so by
+    # symmetry T vanishes at the centre receiver (index 12 of the 5x5 grid)
+    from mtcalc_with_dfp_cocr_amd.receivers import tipper
+
+    Ts = []
+    for k in (0, 1):
+        _, Hx = R.fields(sols[1][k], nu, 1)
+        _, Hy = R.fields(sols[0][k], nu, 0)
+        Ts.append(tipper(R.hz(sols[1][k]), Hx, R.hz(sols[0][k]), Hy))
+    Tg, To = Ts
+    scale = np.abs(To).max()
+    say(f"tipper: max |T| {scale:.3e}, GPU vs oracle {np.abs(Tg - To).max() / scale:.2e}")
+    assert np.abs(Tg - To).max() <= 1e-6 * scale
+    assert np.abs(To[12]).max() <= 1e-4 * scale
 
 
 def test_launcher_end_to_end(M, c1_model, tmp_path):
@@ -828,7 +842,7 @@ def test_full_size_parity(M):
         assert st[k]["req"] == 0 and st[k]["residual_rel"] < 1e-8
         assert abs(st[k]["iter"] - g["iter"]) <= 2 + 0.02 * g["iter"]
         assert r <= bound
-    R = ReceiverOperator(m, np.array(gold["receivers"]))
+    R = ReceiverOperator(m, np.array(gold["receivers"]), smoothing=None)  # as the fixture
     for zg in gold["Z"]:
         f = zg["f"]
         kx = [k for k, c in enumerate(cases) if c == (f, 1)][0]
