@@ -203,8 +203,8 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
  * that runs. Caps change only the reduction grouping of the dot products and the layout only
  * the summation order inside a row (results are deterministic for a fixed shape). cap_vec
  * also sets the fused update pass's cap, which the autotuner tunes apart. Defaults: caps
- * 8192 / 65536 / 2048 (update pass 131072), sliced ELL for 1..2 systems, row blocks for 3..16,
- * row pack for 17..64. */
+ * 8192 / 65536 / 2048 (update pass 131072), sliced ELL for 1..2 systems, the stencil for
+ * 9..16 where it applies, else row blocks for 3..16, row pack for 17..64. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,

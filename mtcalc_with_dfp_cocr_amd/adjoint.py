@@ -55,6 +55,46 @@ def normal_dofs(m: M.Model, nu: float, alpha: int) -> np.ndarray:
     return out[0::2] + 1j * out[1::2]
 
 
+def boundary_edges(m: M.Model) -> np.ndarray:
+    """Dirichlet edges: both end nodes on the outer boundary (model.cpp is_boundary_edge,
+    Code/BuildMatrix/bound_cond_vec_harm.cpp:81-108)."""
+    ma = M.mesh_arrays(m)
+    e = ma.edges.reshape(-1, 2)
+    b = ma.is_node_bound.astype(bool)
+    return b[e[:, 0]] & b[e[:, 1]]
+
+
+def _ks_matrices(m: M.Model):
+    """K and S (full symmetric, host FP64) from the model's K/S arrays (mtmodel_assemble_ks:
+    lower triangle + diagonal, Dirichlet rows at identity in K)."""
+    import scipy.sparse as sp
+
+    p, kd, kg, sd, sg = M.assemble_ks(m)
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+
+    def sym(di, gg):
+        L = sp.csr_matrix((gg, (rows, p.jg)), shape=(p.nb, p.nb))
+        return (L + L.T + sp.diags(di)).tocsr()
+
+    return sym(kd, kg), sym(sd, sg)
+
+
+def s_matrix(m: M.Model):
+    """S(sigma), the sigma-weighted edge mass matrix as the solver sees it (host FP64)."""
+    return _ks_matrices(m)[1]
+
+
+def host_operator(m: M.Model):
+    """(omegas, X) -> rows A(omega_s) x_s = K x_s + i omega_s S x_s on the host in FP64."""
+    K, S = _ks_matrices(m)
+
+    def apply(omegas, X):
+        X = np.atleast_2d(X)
+        return (K @ X.T).T + 1j * np.asarray(omegas)[:, None] * (S @ X.T).T
+
+    return apply
+
+
 def cell_mass_forms(m: M.Model, u: np.ndarray, v: np.ndarray) -> np.ndarray:
     """u_c^T M_c v_c per cell (complex, unconjugated), u and v complex edge vectors."""
     out = np.empty(2 * m.n_cells)
@@ -91,11 +131,12 @@ class GpuBatchSolver:
             self.S.set_gradient(self.ma)
         self._sigma = np.array(sigma, copy=True)
 
-    def solve(self, omegas: np.ndarray, rhs: np.ndarray) -> np.ndarray:
-        """rhs: (nsys, nb) complex. Returns (nsys, nb) complex. Each right-hand side is solved
-        at unit norm and the solution scaled back: the stop test is relative, but the COCR
-        breakdown test on (a, w) is absolute (1e-40, rci.cpp:38) and would fire near a tight
-        tolerance on a small right-hand side."""
+    def solve(self, omegas: np.ndarray, rhs: np.ndarray, eps: float | None = None) -> np.ndarray:
+        """rhs: (nsys, nb) complex. Returns (nsys, nb) complex, solved to eps (default: the
+        solver's). Each right-hand side is solved at unit norm and the solution scaled back:
+        the stop test is relative, but the COCR breakdown test on (a, w) is absolute (1e-40,
+        rci.cpp:38) and would fire near a tight tolerance on a small right-hand side."""
+        eps = self.eps if eps is None else eps
         out = np.empty_like(rhs)
         scale = np.linalg.norm(rhs, axis=1)
         scale[scale == 0] = 1.0
@@ -103,13 +144,13 @@ class GpuBatchSolver:
         for k0 in range(0, len(omegas), 64):
             k1 = min(len(omegas), k0 + 64)
             prs = np.stack([M.to_interleaved(b) for b in rhs[k0:k1]])
-            x, st = self.S.solve_batch(np.asarray(omegas[k0:k1]), prs, eps=self.eps,
+            x, st = self.S.solve_batch(np.asarray(omegas[k0:k1]), prs, eps=eps,
                                        maxiter=self.maxiter, precision="fp64")
             self.stats.extend(st)
             # a breakdown of the (a, w) recurrence once the residual is within 10x the
             # tolerance is convergence at the attainable FP64 level
             bad = [q for q in st if q["req"] != 0 and
-                   not (q["req"] == -2 and q["residual_rel"] < 10 * self.eps)]
+                   not (q["req"] == -2 and q["residual_rel"] < 10 * eps)]
             if bad:
                 raise RuntimeError(f"batched solve did not converge: {bad[0]}")
             out[k0:k1] = x[:, 0::2] + 1j * x[:, 1::2]
@@ -149,23 +190,41 @@ class ImpedanceMisfit:
         self.solver = solver if solver is not None else GpuBatchSolver(m, eps=eps)
         self.a0 = np.stack([np.stack([normal_dofs(m, f, 1), normal_dofs(m, f, 0)])
                             for f in self.freqs])
+        self.interior = ~boundary_edges(m)
+        self.refine_eps = 1e-10
         self.z_obs = z_obs
         self.timings = {}
 
     # ---- forward ---------------------------------------------------------------------------
-    def responses(self, sigma: np.ndarray):
-        """Solutions (nfreq, 2, nb), impedances (nfreq, R, 2, 2) and H (nfreq, R, 2, 2)."""
+    def responses(self, sigma: np.ndarray, base: "State | None" = None):
+        """Solutions (nfreq, 2, nb), impedances (nfreq, R, 2, 2) and H (nfreq, R, 2, 2).
+
+        With base (the state at a nearby sigma_b), each solution is refined out of base.x:
+        x = base.x + d with A(sigma) d = (b(sigma) - b(sigma_b)) - i omega (S(sigma) -
+        S(sigma_b)) base.x, solved to eps relative to that right-hand side. The right-hand
+        side is exactly local to the cells where sigma differs (assembled entries elsewhere are
+        bitwise equal), and x solves A(sigma) x = b(sigma) - r_b, r_b the base's own residual,
+        which two nearby sigma share. A finite difference of J at a step of 1e-4 needs this:
+        solved from scratch, each x holds ~eps |b| of independent Krylov noise, J ~1e-11 J of
+        it, and (J+ - J-) / 2h a floor near 1e-7 whatever the cell's sensitivity. d is solved
+        to refine_eps (1e-10): its relative error multiplies J(sigma) - J(sigma_b), of order
+        |sigma - sigma_b| |dJ|, so a central difference carries ~1e-10 of the cell's own
+        derivative from it."""
         t0 = time.perf_counter()
         m = with_sigma(self.m, sigma)
-        nf = len(self.freqs)
-        rhs = np.empty((nf * 2, self.m.n_edges), complex)
-        for k, f in enumerate(self.freqs):
-            for p, alpha in enumerate((1, 0)):
-                pr = M.assemble_rhs(m, f, alpha)
-                rhs[2 * k + p] = pr[0::2] + 1j * pr[1::2]
+        rhs = self._rhs(m)
         t1 = time.perf_counter()
+        nf = len(self.freqs)
         self.solver.set_sigma(sigma)
-        x = self.solver.solve(np.repeat(self.omegas, 2), rhs).reshape(nf, 2, -1)
+        om = np.repeat(self.omegas, 2)
+        if base is None:
+            x = self.solver.solve(om, rhs).reshape(nf, 2, -1)
+        else:
+            x0 = np.asarray(base.x).reshape(nf * 2, -1)
+            rb, Sb = self._base_parts(base)
+            dS = s_matrix(m) - Sb
+            defect = (rhs - rb) - 1j * om[:, None] * (dS @ x0.T).T
+            x = (x0 + self.solver.solve(om, defect, eps=self.refine_eps)).reshape(nf, 2, -1)
         t2 = time.perf_counter()
         Z = np.empty((nf, self.R.R, 2, 2), complex)
         H = np.empty_like(Z)
@@ -177,6 +236,23 @@ class ImpedanceMisfit:
         self.timings.update(forward_assembly_s=t1 - t0, forward_solve_s=t2 - t1)
         return x, Z, H
 
+    def _rhs(self, m: M.Model) -> np.ndarray:
+        nf = len(self.freqs)
+        rhs = np.empty((nf * 2, self.m.n_edges), complex)
+        for k, f in enumerate(self.freqs):
+            for p, alpha in enumerate((1, 0)):
+                pr = M.assemble_rhs(m, f, alpha)
+                rhs[2 * k + p] = pr[0::2] + 1j * pr[1::2]
+        return rhs
+
+    def _base_parts(self, base: "State"):
+        """b(sigma_b) and S(sigma_b) of a base state, kept for the last base used."""
+        key = getattr(self, "_base_key", None)
+        if key is None or key[0] is not base:
+            mb = with_sigma(self.m, base.sigma)
+            self._base_key = (base, self._rhs(mb), s_matrix(mb))
+        return self._base_key[1], self._base_key[2]
+
     def weights(self):
         O = self.z_obs
         W = np.zeros(O.shape)
@@ -184,8 +260,8 @@ class ImpedanceMisfit:
             W[..., i, j] = 1.0 / np.abs(O[..., i, j]) ** 2
         return W
 
-    def evaluate(self, sigma: np.ndarray) -> State:
-        x, Z, H = self.responses(sigma)
+    def evaluate(self, sigma: np.ndarray, base: State | None = None) -> State:
+        x, Z, H = self.responses(sigma, base)
         W = self.weights()
         J = float(np.sum(W * np.abs(Z - self.z_obs) ** 2))
         return State(np.array(sigma, copy=True), x, self.a0, Z, H, J)
@@ -196,7 +272,7 @@ class ImpedanceMisfit:
         nf, R = len(self.freqs), self.R.R
         W = self.weights() * np.conj(st.Z - self.z_obs)          # (nf, R, 2, 2)
         g = np.zeros((nf, 2, self.m.n_edges), complex)
-        RE, RH = self.R.RE, self.R.RH / 1.0
+        RE, RH = self.R.RE, self.R.RH
         for k in range(nf):
             w = self.omegas[k]
             Hinv = np.linalg.inv(st.H[k])                          # (R, 2, 2)
@@ -207,7 +283,10 @@ class ImpedanceMisfit:
                 ce = (-1j * w) * Q[:, p, :].reshape(-1)            # (2R,)
                 ch = -QZ[:, p, :].reshape(-1)
                 g[k, p] = RE.T @ ce + RH.T @ ch
-        return g
+        # x is held at 0 on the Dirichlet edges, so J varies only with the interior ones: the
+        # receiver maps can read boundary edges (a smoothing subdomain that reaches the outer
+        # boundary), and their part of g would enter lambda through the identity rows
+        return g * self.interior
 
     def gradient(self, st: State) -> np.ndarray:
         """dJ/dsigma per cell."""
@@ -235,8 +314,8 @@ class ImpedanceMisfit:
         self.timings.update(adjoint_solve_s=t1 - t0, gradient_assembly_s=t2 - t1)
         return grad
 
-    def misfit(self, sigma: np.ndarray) -> float:
-        return self.evaluate(sigma).J
+    def misfit(self, sigma: np.ndarray, base: State | None = None) -> float:
+        return self.evaluate(sigma, base).J
 
 
 def torch_misfit(problem: ImpedanceMisfit):
@@ -285,21 +364,39 @@ def config4(seed: int = 4, ncx: int = 64, ncy: int = 64, ncz: int = 40, nrec: in
     return m, rec, freqs, sigma_true
 
 
-def _fd_check(problem: ImpedanceMisfit, sigma: np.ndarray, cells, h: float):
-    """Central differences of J in log(sigma_c) for the given cells."""
+def fd_cells(m: M.Model, rec: np.ndarray, n: int = 20, seed: int = 4) -> np.ndarray:
+    """n cells drawn with the seed among the earth cells under the receiver grid within 600 m
+    of the surface (the cells the surface impedances see)."""
+    area = fd_area(m, rec)
+    return np.random.default_rng(seed).choice(area, size=min(n, len(area)), replace=False)
+
+
+def fd_area(m: M.Model, rec: np.ndarray) -> np.ndarray:
+    xc, yc, zc = m.cell_centres()
+    return np.flatnonzero((zc < 0) & (zc > -600) & (np.abs(xc) < np.abs(rec[:, 0]).max())
+                          & (np.abs(yc) < np.abs(rec[:, 1]).max()))
+
+
+def _fd_check(problem: ImpedanceMisfit, sigma: np.ndarray, cells, h: float,
+              base: State | None = None):
+    """Central differences of J in log(sigma_c) for the given cells. With base (the state at
+    sigma), J(sigma +- h e_c) comes from solutions refined out of base.x
+    (ImpedanceMisfit.responses)."""
     out = []
     for c in cells:
         s1 = sigma.copy()
         s2 = sigma.copy()
         s1[c] *= np.exp(h)
         s2[c] *= np.exp(-h)
-        out.append((problem.misfit(s1) - problem.misfit(s2)) / (2 * h))
+        out.append((problem.misfit(s1, base) - problem.misfit(s2, base)) / (2 * h))
     return np.array(out)
 
 
-def _fd_direction(problem: ImpedanceMisfit, sigma: np.ndarray, d: np.ndarray, h: float):
+def _fd_direction(problem: ImpedanceMisfit, sigma: np.ndarray, d: np.ndarray, h: float,
+                  base: State | None = None):
     """Central difference of J along the log(sigma) direction d (per cell)."""
-    return (problem.misfit(sigma * np.exp(h * d)) - problem.misfit(sigma * np.exp(-h * d))) / (2 * h)
+    return (problem.misfit(sigma * np.exp(h * d), base)
+            - problem.misfit(sigma * np.exp(-h * d), base)) / (2 * h)
 
 
 def main(argv=None):
@@ -331,25 +428,19 @@ def main(argv=None):
     t2 = time.perf_counter()
     grad = prob.gradient(st) * sigma
     t3 = time.perf_counter()
-    # FD cells among the earth cells under the receiver area, within 600 m of the surface: half
-    # the most sensitive ones, half seeded among those whose sensitivity is above the finite
-    # differences' noise floor (|dJ/dlog sigma| >= 1e-2 of the largest: the solves stop at eps,
-    # so J carries ~eps J of noise, ~1e-6 J after / 2h). Then whole random +-1 directions over
-    # all those cells, whose derivative gathers every cell's sensitivity.
-    xc, yc, zc = m.cell_centres()
-    area = np.flatnonzero((zc < 0) & (zc > -600) & (np.abs(xc) < np.abs(rec[:, 0]).max())
-                          & (np.abs(yc) < np.abs(rec[:, 1]).max()))
-    top = area[np.argsort(-np.abs(grad[area]))[: a.fd_cells // 2]]
-    cand = np.setdiff1d(area[np.abs(grad[area]) >= 1e-2 * np.abs(grad).max()], top)
+    # FD cells drawn with the seed among the earth cells under the receiver area within 600 m
+    # of the surface, each J(sigma +- h e_c) refined out of the solutions at sigma by defect
+    # correction; then whole random +-1 directions over all those cells, whose derivative
+    # gathers every cell's sensitivity.
+    area = fd_area(m, rec)
+    cells = fd_cells(m, rec, a.fd_cells, a.seed)
+    fd = _fd_check(prob, sigma, cells, a.h, st)
     rng = np.random.default_rng(a.seed + 7)
-    pick = rng.choice(cand, size=min(a.fd_cells - len(top), len(cand)), replace=False)
-    cells = np.concatenate([top, pick])
-    fd = _fd_check(prob, sigma, cells, a.h)
     dirs = []
     for _ in range(a.directions):
         d = np.zeros_like(sigma)
         d[area] = rng.choice([-1.0, 1.0], size=len(area))
-        fdd = _fd_direction(prob, sigma, d, a.h)
+        fdd = _fd_direction(prob, sigma, d, a.h, st)
         add = float(grad @ d)
         dirs.append({"adjoint": add, "fd": fdd, "rel_err": abs(add - fdd) / abs(fdd)})
     t4 = time.perf_counter()

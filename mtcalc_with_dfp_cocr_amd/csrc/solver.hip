@@ -1994,9 +1994,10 @@ struct mtcocr_handle {
       }
   }
   mtcocr_handle() {
-    // row blocks for 4..16 systems, the row pack for 32 and 64 (set_caps clamps layout 2 there);
+    // the stencil for 16 systems where the matrix has it, else row blocks for 4..16 systems
+    // where their halos fit, else the row pack (set_caps clamps the layout for 32 and 64);
     // streaming kernels at about one tile per workgroup (the grid is clamped to the tiles)
-    set_caps(8192, 65536, 2048, 2);
+    set_caps(8192, 65536, 2048, 3);
     for (auto& row : cap_upd)
       for (int& c : row) c = kMaxBlocks;  // the update pass: one tile per workgroup
     for (auto& row : stc_kc)

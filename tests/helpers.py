@@ -131,7 +131,7 @@ class DirectKSSolver:
         self._lu = {}
         self._sigma = np.array(sigma, copy=True)
 
-    def solve(self, omegas, rhs):
+    def solve(self, omegas, rhs, eps=None):
         import scipy.sparse.linalg as spl
 
         out = np.empty_like(rhs)

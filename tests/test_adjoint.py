@@ -59,6 +59,23 @@ def test_normal_dofs_build_the_rhs(M, small_model):
                                    atol=1e-12 * np.abs(ref).max())
 
 
+def test_boundary_edges_are_the_dirichlet_rows(M, small_model):
+    """The adjoint's Dirichlet edges are the rows the assembly holds at identity: K = 1 and
+    S = 0 on the diagonal and no off-diagonal entry in their rows or columns."""
+    from mtcalc_with_dfp_cocr_amd.adjoint import boundary_edges
+
+    p, kd, kg, sd, sg = M.assemble_ks(small_model)
+    bnd = boundary_edges(small_model)
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+    touched = np.zeros(p.nb, bool)
+    nzk = (kg != 0) | (sg != 0)
+    touched[rows[nzk]] = True
+    touched[p.jg[nzk]] = True
+    assert bnd.sum() > 0
+    assert np.all(kd[bnd] == 1.0) and np.all(sd[bnd] == 0.0) and not touched[bnd].any()
+    assert np.all(touched[~bnd] | (sd[~bnd] != 0))
+
+
 def test_cell_mass_forms_sum_to_s(M, small_model):
     """sum_c sigma_c u_c^T M_c v_c = u^T S v (interior rows and columns)."""
     from mtcalc_with_dfp_cocr_amd.adjoint import cell_mass_forms
@@ -92,6 +109,56 @@ def test_gradient_matches_finite_differences(problem):
         s2[c] *= np.exp(-h)
         fd = (problem.misfit(s1) - problem.misfit(s2)) / (2 * h)
         assert grad[c] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())
+
+
+def test_defect_correction_reproduces_the_solution(problem):
+    """responses(sigma', base) = x0 + A'^-1 [(b' - b) - i omega (S' - S) x0] solves
+    A' x = b' - r0 (r0 the base's residual, at LU's level here): with exact LU solves, at a
+    perturbed cell and along a whole direction, its true residual (from the solver's own K and
+    S) is at LU's level and its impedances and J equal those of the direct solution. (x
+    itself is not compared: in the air, sigma = 1e-8 leaves a near-null space of gradient
+    fields that round-off moves without changing any field at the surface.)"""
+    from mtcalc_with_dfp_cocr_amd import adjoint as AD
+
+    sigma = problem.m.sigma.copy()
+    st = problem.evaluate(sigma)
+    xc, yc, zc = problem.m.cell_centres()
+    earth = np.flatnonzero(zc < 0)
+    s1 = sigma.copy()
+    s1[earth[len(earth) // 2]] *= np.exp(1e-3)
+    s2 = sigma * np.exp(0.05 * np.random.default_rng(2).standard_normal(len(sigma)))
+    om = np.repeat(problem.omegas, 2)
+
+    def true_rel(s, x):
+        m = AD.with_sigma(problem.m, s)
+        problem.solver.set_sigma(s)
+        K, S = problem.solver.K, problem.solver.S
+        out = []
+        for k, f in enumerate(problem.freqs):
+            for p, alpha in enumerate((1, 0)):
+                pr = problem.solver.M.assemble_rhs(m, f, alpha)
+                b = pr[0::2] + 1j * pr[1::2]
+                xs = x[k, p]
+                out.append(np.linalg.norm(b - K @ xs - 1j * om[2 * k] * (S @ xs))
+                           / np.linalg.norm(b))
+        return max(out)
+
+    for s in (s1, s2):
+        x_direct, Z_direct, _ = problem.responses(s)
+        x_dc, Z_dc, _ = problem.responses(s, st)
+        assert true_rel(s, x_dc) <= max(1e-11, 2 * true_rel(s, x_direct))
+        np.testing.assert_allclose(Z_dc, Z_direct, rtol=1e-9, atol=1e-9 * np.abs(Z_direct).max())
+        assert problem.misfit(s, st) == pytest.approx(problem.misfit(s), rel=1e-10)
+    # the right-hand side without the (S' - S) x0 term leaves A' x = b' + (A' - A) x0
+    good = AD.s_matrix
+    try:
+        problem._base_key = None
+        AD.s_matrix = lambda m: 0.0 * good(m)
+        x_bad, _, _ = problem.responses(s2, st)
+    finally:
+        AD.s_matrix = good
+        problem._base_key = None
+    assert true_rel(s2, x_bad) > 1e-3
 
 
 def test_torch_wrapper(problem):

@@ -1,0 +1,189 @@
+"""The benchmark configurations that the headline run does not cover, at their full sizes, on
+the GPU through the C-ABI:
+
+  config 3  four seeded models (32^3, 48^3, 64x64x48, 96x96x64 cells), 16 frequencies from
+            1e-3 to 1e3 Hz x 2 polarizations each: one batch of 32 systems per model, in mixed
+            precision (FP32 inner COCR, FP64 refinement) and in FP64;
+  config 4  the adjoint gradient of the impedance misfit on 64x64x40 cells against central
+            finite differences (h = 1e-4 in log sigma) on 20 cells drawn from the seed;
+  config 5  256x256x128 cells (25.4 M edges), layered background and 10 ellipsoids, 4
+            frequencies from 1e-2 to 1e1 Hz x 2 polarizations: one batch of 8, mixed and FP64.
+
+Residual bounds. The FP64 solver stops, as the reference does, on the MRS-smoothed residual; the
+raw iterate it returns sits above eps by an amount that rounding decides. The oracle's own
+returned x reaches 1.06e-7 on the full-size benchmark model (tests/golden/config2_oracle.json
+and its README) and, on system 19 of config 3's 48^3 model (3.98 Hz, Y), 1.65e-7 on one thread
+and 5.5e-8 on eight, where the GPU's (the same 683 iterations) reaches 2.9e-7. FP64 answers are
+therefore held to a true relative residual of 5e-7, a bound on the method's attainable accuracy
+at the stop point rather than on eps, next to the stop test itself (MRS residual < eps) and the
+impedances. The mixed solver's refinement tests the true FP64 residual itself, so its answers
+are held to eps = 1e-8.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+FP64_TRUE_BOUND = 5e-7
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def ks_operator(p, kd, kg, sd, sg):
+    """A(omega) x = K x + i omega S x from the lower-triangle K/S arrays (host, FP64)."""
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+
+    def sym(di, gg):
+        L = sp.csr_matrix((gg, (rows, p.jg)), shape=(p.nb, p.nb))
+        return (L + L.T + sp.diags(di)).tocsr()
+
+    K, S = sym(kd, kg), sym(sd, sg)
+
+    def true_rel(pr, omega, x):
+        b = pr[0::2] + 1j * pr[1::2]
+        xc = x[0::2] + 1j * x[1::2]
+        return float(np.linalg.norm(b - (K @ xc + 1j * omega * (S @ xc))) / np.linalg.norm(b))
+
+    return true_rel
+
+
+def solve_model(M, m, freqs, precisions, order=-1):
+    from mtcalc_with_dfp_cocr_amd.cocr import Solver
+
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(m)
+    ma = M.mesh_arrays(m)
+    prs = np.stack([M.assemble_rhs(m, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    out = {}
+    with Solver(0) as S:
+        S.set_mesh_order(ma, order)
+        S.set_matrix_ks(p, kd, kg, sd, sg)
+        S.set_gradient(ma)
+        for prec in precisions:
+            x, st = S.solve_batch(om, prs, eps=1e-8, maxiter=40000, precision=prec)
+            out[prec] = (x, st)
+        info = S.info()
+    return cases, prs, om, ks_operator(p, kd, kg, sd, sg), out, ma, info
+
+
+@pytest.mark.timeout(900)
+def test_config3_four_models(O, M):
+    """Config 3: every system of the four models converges in mixed precision (true FP64
+    residual < 1e-8) and in FP64 (stop test met, true residual <= 5e-7); the mixed and FP64 answers give the
+    same impedances at a 5x5 grid to 1e-6; on the 32^3 model, two frequencies are solved by the
+    oracle too, with the iteration counts and impedances compared."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.receivers import ReceiverOperator, impedance
+
+    shapes = [(32, 32, 32), (48, 48, 48), (64, 64, 48), (96, 96, 64)]
+    freqs = M.frequencies(1e-3, 1e3, 16)
+    g = np.arange(-200.0, 201.0, 100.0)
+    rx, ry = np.meshgrid(g, g, indexing="ij")
+    rec = np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1)
+
+    def Z(R, x, f, kx):
+        Ex, Hx = R.fields(M.to_complex(x[kx]), f, 1)
+        Ey, Hy = R.fields(M.to_complex(x[kx + 1]), f, 0)
+        return impedance(Ex, Hx, Ey, Hy)
+
+    def zdiff(Za, Zb):
+        d = np.abs(Za - Zb) / np.abs(Zb[:, 0, 1])[:, None, None]
+        d[:, 1, 0] = np.abs(Za[:, 1, 0] - Zb[:, 1, 0]) / np.abs(Zb[:, 1, 0])
+        return float(d.max())
+
+    for k, (nx, ny, nz) in enumerate(shapes):
+        m = M.random_model(nx, ny, nz, seed=k, h=50.0, name=f"c3_{nx}x{ny}x{nz}")
+        say(f"config 3 model {m.name}: nb = {m.n_edges}, 32 systems")
+        cases, prs, om, true_rel, out, ma, info = solve_model(M, m, freqs, ("mixed", "fp64"))
+        xm, sm = out["mixed"]
+        xd, sd_ = out["fp64"]
+        for s in range(len(cases)):
+            rm, rd = true_rel(prs[s], om[s], xm[s]), true_rel(prs[s], om[s], xd[s])
+            assert sm[s]["req"] == 0 and rm < 1e-8, (m.name, s, sm[s], rm)
+            assert sd_[s]["req"] == 0 and sd_[s]["residual_rel"] < 1e-8
+            assert rd <= FP64_TRUE_BOUND, (m.name, s, sd_[s], rd)
+        say(f"  fp64 iterations {[q['iter'] for q in sd_]}; mixed passes "
+            f"{[q['outer'] for q in sm]}, FP64 finishes {sum(q['flags'] & 1 for q in sm)}")
+        R = ReceiverOperator(m, rec)
+        worst = max(zdiff(Z(R, xm, f, 2 * i), Z(R, xd, f, 2 * i)) for i, f in enumerate(freqs))
+        say(f"  Z mixed vs fp64: max rel diff {worst:.2e}")
+        assert worst <= 1e-6
+        if k == 0:
+            for i in (3, 12):  # 6.3e-3 Hz and 63 Hz, both polarizations, against the oracle
+                xo = []
+                for kk in (2 * i, 2 * i + 1):
+                    s = M.assemble(m, *cases[kk])
+                    x, so, _ = O.pcocr(s, ma, eps=1e-8, nthreads=8)
+                    say(f"  oracle {cases[kk][0]:.3g} Hz pol {'XY'[1 - cases[kk][1]]}: "
+                        f"{so['iter']} it, GPU {sd_[kk]['iter']} it")
+                    assert abs(sd_[kk]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+                    xo.append(x)
+                zo = Z(R, np.array(xo), freqs[i], 0)
+                assert zdiff(Z(R, xd, freqs[i], 2 * i), zo) <= 1e-6
+
+
+@pytest.mark.timeout(900)
+def test_config4_adjoint_vs_finite_differences(M):
+    """Config 4 (64x64x40 cells, 15x15 receivers, 6 frequencies, Zxy + Zyx): the adjoint
+    gradient in log sigma against central differences (h = 1e-4) on 20 cells drawn from the
+    seed among the earth cells under the receiver grid within 600 m of the surface, per cell
+    |fd - adjoint| <= 1e-4 |adjoint| + 1e-9 J. The sensitivities of those cells span 1e-4 ..
+    0.2 (J = 1.5), so J(sigma +- h e_c) must be resolved to ~1e-13 J: each is evaluated from
+    solutions refined out of the solutions at sigma by defect correction, which leaves the
+    solver's error on the defect's scale. Solved from scratch at eps = 1e-12, the same
+    differences carried a floor of 1e-8 .. 4.5e-7 on every cell."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd import adjoint as AD
+
+    m, rec, freqs, sigma_true = AD.config4()
+    say(f"config 4: nb = {m.n_edges}, {len(freqs)} freqs x 2 pol, {len(rec)} receivers")
+    prob = AD.ImpedanceMisfit(m, freqs, rec, eps=1e-12)
+    try:
+        _, zobs, _ = prob.responses(sigma_true)
+        prob.z_obs = zobs
+        sigma = m.sigma.copy()
+        st = prob.evaluate(sigma)
+        grad = prob.gradient(st) * sigma
+        cells = AD.fd_cells(m, rec, 20, seed=4)
+        fd = AD._fd_check(prob, sigma, cells, 1e-4, st)
+    finally:
+        prob.solver.close()
+    ad = grad[cells]
+    err = np.abs(fd - ad)
+    say(f"J = {st.J:.6e}; |adjoint| {np.abs(ad).min():.2e}..{np.abs(ad).max():.2e}; "
+        f"rel err max {np.max(err / np.abs(ad)):.2e}")
+    for c, a, f, e in zip(cells, ad, fd, err):
+        say(f"  cell {c}: adjoint {a:.8e} fd {f:.8e} rel {e / abs(a):.2e}")
+    assert np.all(err <= 1e-4 * np.abs(ad) + 1e-9 * st.J)
+    assert np.linalg.norm(fd - ad) <= 1e-4 * np.linalg.norm(ad)
+
+
+@pytest.mark.timeout(1200)
+def test_config5_large_batch(M):
+    """Config 5 (25.4 M edges, 8 systems): one batch in mixed precision and one in FP64, every
+    system converged (true residual < 1e-8 mixed, <= 5e-7 FP64) and the device footprint
+    reported against the 288 GB of one MI355X."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    m = M.ellipsoid_model(256, 256, 128, seed=0)
+    freqs = M.frequencies(1e-2, 1e1, 4)
+    say(f"config 5: nb = {m.n_edges}, 8 systems")
+    cases, prs, om, true_rel, out, ma, info = solve_model(M, m, freqs, ("mixed", "fp64"),
+                                                          order=16)
+    say(f"  device bytes {info['device_bytes'] / 1e9:.1f} GB of 288")
+    assert info["device_bytes"] < 288e9
+    for prec, bound in (("mixed", 1e-8), ("fp64", FP64_TRUE_BOUND)):
+        x, st = out[prec]
+        for s in range(len(cases)):
+            r = true_rel(prs[s], om[s], x[s])
+            say(f"  {prec} {cases[s][0]:.3g} Hz pol {'XY'[1 - cases[s][1]]}: req {st[s]['req']}"
+                f" it {st[s]['iter']} true rel res {r:.2e} ({st[s]['seconds']:.1f} s batch)")
+            assert st[s]["req"] == 0 and r <= bound
