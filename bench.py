@@ -186,7 +186,7 @@ def cpu_solve(M, m, ma, case, threads):
 
 # SpMV kernel of each layout name the autotuner reports
 SPMV_KERNELS = {"ell": "k_spmv", "rowpack": "k_spmv_rg", "rowblock": "k_spmv_blk",
-                "stencil": "k_spmv_stc"}
+                "stencil": "k_spmv_stc", "stencil_lds": "k_spmv_stl"}
 
 
 def measure_traffic(args, sp):
@@ -205,7 +205,7 @@ def measure_traffic(args, sp):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    pats = {"spmv": re.compile(r"k_spmv(_rg|_blk|_stc)?<double, %d, 0>" % sp),
+    pats = {"spmv": re.compile(r"k_spmv(_rg|_blk|_stc|_stl)?<double, %d, 0>" % sp),
             "update": re.compile(r"k_xr_fused<double, %d>" % sp)}
     res = {"spmv": {}, "update": {}}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):

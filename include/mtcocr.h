@@ -122,8 +122,8 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
  * rows fit the edge-element stencil (33 couplings at fixed offsets) also gets the structured
  * stencil SpMV layout (mtcocr_set_launch layout 3). Per-row arithmetic is unchanged (each row keeps its entry order), dot
  * products regroup. Returns MTCOCR_ERR_ARG, keeping the caller's order, for a mesh that is not
- * structured hexahedral. mtcocr_pcocr_2x2_folded and the COCR_FP executable apply it with
- * block 16. */
+ * structured hexahedral. mtcocr_pcocr_2x2_folded and the COCR_FP executable apply the grid
+ * order where the mesh is a box, else block 16. */
 int mtcocr_set_mesh_order(mtcocr_handle* h, int kpar, int n_nodes, const int* nver, int n_edges,
                           const int* edges, int block);
 
@@ -195,28 +195,41 @@ int mtcocr_bench_spmv(mtcocr_handle* h, int precision, int nsys, int reps, doubl
  *     3..16 systems, when no block has more than 192 distinct columns); where it does not
  *     apply, the row pack runs;
  *   3 structured stencil: 42 coefficients per row at fixed stencil positions, each wave
- *     sweeping a column of rows along z (batches of 3..16 systems, matrix set in the grid order
- *     and fitting the stencil); where it does not apply, the row blocks run.
+ *     sweeping a column of rows along z (batches of 9..16 systems, matrix set in the grid order
+ *     and fitting the stencil); where it does not apply, the row blocks run;
+ *   4 the same stencil with x staged in LDS: each workgroup sweeps a 4 x 4 tile of rows and
+ *     copies the tile's x neighbourhood of every level into LDS once (same conditions; each
+ *     row's result equals layout 3's bit for bit).
  * The row layouts need the batch vectors under 4 GiB (nb x width x 16 bytes in FP64); above
  * that the sliced ELL runs. set_launch applies to both precisions and every batch width;
  * get_launch reports those of `precision` at the current batch width, the layout as the one
  * that runs. Caps change only the reduction grouping of the dot products and the layout only
  * the summation order inside a row (results are deterministic for a fixed shape). cap_vec
  * also sets the fused update pass's cap, which the autotuner tunes apart. Defaults: caps
- * 8192 / 65536 / 2048 (update pass 131072), sliced ELL for 1..2 systems, the stencil for
- * 9..16 where it applies, else row blocks for 3..16, row pack for 17..64. */
+ * 8192 / 65536 / 2048 (update pass 131072), sliced ELL for 1..2 systems, the LDS-staged
+ * stencil (layout 4; layout 3 is the wave-task stencil) for 9..16 where it applies, else row
+ * blocks for 3..16, row pack for 17..64. */
 int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
                       int spmv_layout);
 int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_vec,
                       int* cap_node, int* spmv_layout);
 
-/* Sweep length of the stencil SpMV (layout 3): rows of one edge column that a wave task
- * sweeps along z (at least 1; default 16; longer sweeps reload fewer levels, shorter ones keep
- * the concurrently swept region of x within the L2 cache). set applies to every width and
- * precision; get reports the value at the current batch width. No counterpart in the
- * reference. */
+/* Sweep length of the stencil SpMVs (layouts 3 and 4): rows of one edge column that a task
+ * sweeps along z (at least 1; default: whole columns; longer sweeps reload fewer levels,
+ * shorter ones make more tasks and keep the concurrently swept region of x within the L2
+ * cache). set applies to every width and precision; get reports the value at the current
+ * batch width. No counterpart in the reference. */
 int mtcocr_set_stencil_sweep(mtcocr_handle* h, int kc);
 int mtcocr_get_stencil_sweep(mtcocr_handle* h, int precision, int* kc);
+
+/* Task order of the LDS-staged stencil SpMV (layout 4): bands of `band` tile columns (4 rows
+ * each in x), each band's tiles row by row, so that the workgroups of one XCD work on a patch
+ * whose tiles share their x halos within one L2; 0 (the default) orders all tile columns as
+ * one band. Only the order changes: every product is the same, the dot products' partial
+ * sums regroup. set applies to every width and precision; get reports the value at the
+ * current batch width. The autotuner searches it. No counterpart in the reference. */
+int mtcocr_set_stencil_band(mtcocr_handle* h, int band);
+int mtcocr_get_stencil_band(mtcocr_handle* h, int precision, int* band);
 
 /* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),

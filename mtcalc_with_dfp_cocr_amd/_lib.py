@@ -95,6 +95,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_int)]),
     "mtcocr_set_stencil_sweep": (C.c_int, [C.c_void_p, C.c_int]),
     "mtcocr_get_stencil_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "mtcocr_set_stencil_band": (C.c_int, [C.c_void_p, C.c_int]),
+    "mtcocr_get_stencil_band": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mtcocr_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),

@@ -176,8 +176,8 @@ class Solver:
               "mtcocr_bench_copy")
         return ms.value
 
-    LAYOUTS = {"ell": 0, "rowpack": 1, "rowblock": 2, "stencil": 3}
-    LAYOUT_NAMES = ("ell", "rowpack", "rowblock", "stencil")
+    LAYOUTS = {"ell": 0, "rowpack": 1, "rowblock": 2, "stencil": 3, "stencil_lds": 4}
+    LAYOUT_NAMES = ("ell", "rowpack", "rowblock", "stencil", "stencil_lds")
 
     def set_launch(self, cap_spmv: int = 0, cap_vec: int = 0, cap_node: int = 0,
                    spmv_layout: str | None = None) -> None:
@@ -195,8 +195,18 @@ class Solver:
                 "spmv_layout": self.LAYOUT_NAMES[d.value]}
 
     def set_stencil_sweep(self, kc: int) -> None:
-        """Rows a wave task of the stencil SpMV sweeps along z (every width and precision)."""
+        """Rows a task of the stencil SpMVs sweeps along z (every width and precision)."""
         check(self._L.mtcocr_set_stencil_sweep(self._h, int(kc)), "mtcocr_set_stencil_sweep")
+
+    def set_stencil_band(self, band: int) -> None:
+        """Tile columns per band of the LDS-staged stencil SpMV's task order (0: one band)."""
+        check(self._L.mtcocr_set_stencil_band(self._h, int(band)), "mtcocr_set_stencil_band")
+
+    def get_stencil_band(self, precision: str = "fp64") -> int:
+        b = C.c_int()
+        check(self._L.mtcocr_get_stencil_band(self._h, PRECISION[precision], C.byref(b)),
+              "mtcocr_get_stencil_band")
+        return b.value
 
     def get_stencil_sweep(self, precision: str = "fp64") -> int:
         kc = C.c_int()
@@ -213,8 +223,10 @@ class Solver:
               "mtcocr_autotune")
         out = {"cap_spmv": a.value, "cap_vec": b.value, "cap_node": c.value,
                "spmv_layout": self.LAYOUT_NAMES[d.value], "spmv_ms": ms.value}
-        if out["spmv_layout"] == "stencil":
+        if out["spmv_layout"] in ("stencil", "stencil_lds"):
             out["stencil_sweep"] = self.get_stencil_sweep(precision)
+        if out["spmv_layout"] == "stencil_lds":
+            out["stencil_band"] = self.get_stencil_band(precision)
         return out
 
     def apply(self, op: str, x, precision: str = "fp64"):

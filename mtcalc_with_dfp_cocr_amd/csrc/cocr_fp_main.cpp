@@ -242,11 +242,15 @@ int set_gradient(mtcocr_handle* h, Inputs& in) {
                                   in.tsize3d ? in.gg_t.data() : nullptr, in.is_node_bound.data());
 }
 
-// the device's locality order of the mesh (kept as given when the mesh is not a structured
-// hexahedral one or has hanging nodes)
+// the device's locality order of the mesh: the grid order of a box mesh (which also enables
+// the stencil SpMV for batches), else the pencil order of a structured hexahedral one, else the
+// order given (hanging nodes)
 void set_order(mtcocr_handle* h, Inputs& in) {
   if (in.tsize3d) return;
-  (void)mtcocr_set_mesh_order(h, in.kpar, in.kuzlov, in.nver.data(), in.n_c, in.edges.data(), 16);
+  if (mtcocr_set_mesh_order(h, in.kpar, in.kuzlov, in.nver.data(), in.n_c, in.edges.data(),
+                            MTCOCR_ORDER_GRID) != MTCOCR_OK)
+    (void)mtcocr_set_mesh_order(h, in.kpar, in.kuzlov, in.nver.data(), in.n_c, in.edges.data(),
+                                16);
 }
 
 int run_single(const std::string& dir, bool mixed, bool strict) {

@@ -988,7 +988,10 @@ struct StcView {
   int nx, ny, nz;      // nodes per axis
   int kc;              // rows of one sweep (k-chunk)
   int nic;             // i-chunks of RPW rows
-  long long ntasks;    // 3 * nkc * ny * nic wave tasks
+  long long ntasks;    // 3 * nkc * ny * nic wave tasks (k_spmv_stc), 3 * nkc * njc * nic
+                       // workgroup tasks (k_spmv_stl)
+  int njc;             // j-chunks of kStlNW rows (k_spmv_stl)
+  int nbw;             // k_spmv_stl: tile columns per band of the task order
   int nb;
 };
 
@@ -1027,6 +1030,45 @@ __device__ inline void stc_store(bool ok, unsigned off, double yr, double yi,
     const double rr = double(out.x) * double(out.x) + double(out.y) * double(out.y);
     v[1] += ok ? rr : 0.0;
     const T2 xt = buffer_gather<T2>(xr, o);
+    store(ym_r, xt);
+    store(sm_r, out);
+  }
+}
+
+// stc_store with the row's own x given (the LDS-staged kernel has it from the previous level)
+// instead of loaded: the same outputs and partial sums, bit for bit
+template <class T, int MODE>
+__device__ inline void stl_store(bool ok, unsigned off, double yr, double yi,
+                                 typename Vec2<T>::type xself, __amdgpu_buffer_rsrc_t y_r,
+                                 __amdgpu_buffer_rsrc_t b_r, __amdgpu_buffer_rsrc_t ym_r,
+                                 __amdgpu_buffer_rsrc_t sm_r, double (&v)[2]) {
+  using T2 = typename Vec2<T>::type;
+  const unsigned o = ok ? off : kOob;
+  auto store = [&](__amdgpu_buffer_rsrc_t r, T2 val) {
+    if constexpr (sizeof(T2) == 16)
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, val), r, int(o), 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, val), r, int(o), 0, 0);
+  };
+  T2 xt;
+  xt.x = ok ? xself.x : T(0);
+  xt.y = ok ? xself.y : T(0);
+  if (MODE == 0) {
+    T2 out;
+    out.x = T(yr);
+    out.y = T(yi);
+    store(y_r, out);
+    v[0] += double(out.x) * double(xt.x) - double(out.y) * double(xt.y);
+    v[1] += double(out.x) * double(xt.y) + double(out.y) * double(xt.x);
+  } else {
+    const T2 bv = buffer_gather<T2>(b_r, o);
+    T2 out;
+    out.x = T(double(bv.x) - yr);
+    out.y = T(double(bv.y) - yi);
+    store(y_r, out);
+    v[0] += double(bv.x) * double(bv.x) + double(bv.y) * double(bv.y);
+    const double rr = double(out.x) * double(out.x) + double(out.y) * double(out.y);
+    v[1] += ok ? rr : 0.0;
     store(ym_r, xt);
     store(sm_r, out);
   }
@@ -1213,6 +1255,280 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stc(StcView A, const double* om
       stc_sweep<T, SP, MODE, 1>(A, ic, j, kc, s, g, w, run, xr, cr, y_r, b_r, ym_r, sm_r, v);
     else
       stc_sweep<T, SP, MODE, 2>(A, ic, j, kc, s, g, w, run, xr, cr, y_r, b_r, ym_r, sm_r, v);
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// LDS-staged stencil SpMV (layout 4, 16 systems). A workgroup of kStlNW waves owns a tile of
+// rows of one direction: RPW = 4 consecutive rows in i (wave-wide, as in k_spmv_stc) times
+// kStlNW consecutive rows in j (one per wave), and sweeps a chunk of k. At each level L the
+// tile's x neighbourhood at L (per column direction dt, the rectangle that the tile's columns
+// reach) is copied into LDS once, by all 256 threads with coalesced 16-byte loads, and every
+// wave then reads its 13 (21) column values from LDS and pushes them into rows L-1..L+1 as
+// k_spmv_stc does. Each x value crosses L2 -> CU once per tile and level instead of once per
+// column and row group. Two plane buffers: plane L+1 is loaded into registers while level L
+// computes and written to the other buffer afterwards, with one workgroup barrier per level.
+constexpr int kStlNW = kBlock / 64;  // waves per workgroup = rows of a tile in j
+constexpr int kWholeColumn = 1 << 20;  // a stencil sweep length past any mesh's nz
+constexpr int kStlTI = 4;            // rows of a tile in i (64 / 16 lanes per row)
+constexpr int stl_imin(int a, int b) { return a < b ? a : b; }
+constexpr int stl_imax(int a, int b) { return a > b ? a : b; }
+// range of di (which 0, 1) or dj (2, 3) over the columns of a type-D row that read dt
+constexpr int stl_rng(int D, int dt, int which) {
+  int r = which % 2 == 0 ? 99 : -99;
+  for (int c = 0; c < kStcCols[D]; ++c)
+    if (kStc[D][c].dt == dt) {
+      const int v = which < 2 ? kStc[D][c].di : kStc[D][c].dj;
+      r = which % 2 == 0 ? stl_imin(r, v) : stl_imax(r, v);
+    }
+  return r;
+}
+constexpr int stl_wid(int D, int dt) { return kStlTI + stl_rng(D, dt, 1) - stl_rng(D, dt, 0); }
+constexpr int stl_hgt(int D, int dt) { return kStlNW + stl_rng(D, dt, 3) - stl_rng(D, dt, 2); }
+constexpr int stl_base(int D, int dt) {
+  int b = 0;
+  for (int q = 0; q < dt; ++q) b += stl_wid(D, q) * stl_hgt(D, q);
+  return b;
+}
+constexpr int stl_nval(int D) { return stl_base(D, 3); }
+// the column of a type-D row's own edge (dt = D, di = dj = 0)
+constexpr int stl_self(int D) {
+  for (int c = 0; c < kStcCols[D]; ++c)
+    if (kStc[D][c].dt == D && kStc[D][c].di == 0 && kStc[D][c].dj == 0) return c;
+  return -1;
+}
+static_assert(stl_self(0) >= 0 && stl_self(1) >= 0 && stl_self(2) >= 0, "own column");
+constexpr int kStlMaxVal = stl_imax(stl_nval(0), stl_imax(stl_nval(1), stl_nval(2)));
+static_assert(stl_nval(0) == 79 && stl_nval(1) == 79 && stl_nval(2) == 96,
+              "plane sizes of a 4 x 4 tile");
+
+template <class T, int SP, int MODE, int D>
+__device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s, int g, int wv,
+                                 double w, bool run, __amdgpu_buffer_rsrc_t xr,
+                                 __amdgpu_buffer_rsrc_t cr, __amdgpu_buffer_rsrc_t yr_,
+                                 __amdgpu_buffer_rsrc_t br_, __amdgpu_buffer_rsrc_t ymr,
+                                 __amdgpu_buffer_rsrc_t smr, typename Vec2<T>::type* pl,
+                                 double (&v)[2]) {
+  using T2 = typename Vec2<T>::type;
+  constexpr int NC = kStcCols[D];
+  constexpr int NCR = (kStcCoef + SP - 1) / SP;
+  constexpr int LO = D == 2 ? 0 : -1, HI = 1;
+  constexpr int PS = kStlMaxVal * SP;              // elements of one plane buffer
+  const GridDims G{A.nx, A.ny, A.nz};
+  const int nxd = G.nxd(D), nyd = G.nyd(D), nzd = G.nzd(D);
+  const int i0 = ic * kStlTI, j0 = jc * kStlNW;
+  const int k0 = kc * A.kc, k1 = min(k0 + A.kc, nzd);
+  if (j0 >= nyd || k0 >= nzd || i0 >= nxd) return;  // the whole tile is empty (uniform)
+  const int i = i0 + g, j = j0 + wv;
+  const bool row_ok = i < nxd && j < nyd;
+  const int ir = min(i, nxd - 1), jr = min(j, nyd - 1);
+  // this thread's share of a plane, per column direction dt (so that dt is a compile-time
+  // constant of each load): element q = t + kBlock*m of dt's rectangle is system q % SP of its
+  // value q / SP; the last load of each dt partly empty (offset kOob: it returns 0 and its
+  // value is not stored)
+  const int t = int(threadIdx.x);
+  constexpr int NL0 = (stl_wid(D, 0) * stl_hgt(D, 0) * SP + kBlock - 1) / kBlock;
+  constexpr int NL1 = (stl_wid(D, 1) * stl_hgt(D, 1) * SP + kBlock - 1) / kBlock;
+  constexpr int NL2 = (stl_wid(D, 2) * stl_hgt(D, 2) * SP + kBlock - 1) / kBlock;
+  constexpr int NL = NL0 + NL1 + NL2;
+  constexpr int kFirst[3] = {0, NL0, NL0 + NL1};
+  unsigned lo[NL];
+  static_for<0, NL>([&](auto Mc) {
+    constexpr int m = decltype(Mc)::value;
+    constexpr int dt = m < NL0 ? 0 : m < NL0 + NL1 ? 1 : 2;
+    constexpr int n = stl_wid(D, dt) * stl_hgt(D, dt) * SP;
+    const int q = t + kBlock * (m - kFirst[dt]);
+    const int val = min(q, n - 1) / SP, sys = q % SP;
+    const int gi = min(max(i0 + stl_rng(D, dt, 0) + val % stl_wid(D, dt), 0), G.nxd(dt) - 1);
+    const int gj = min(max(j0 + stl_rng(D, dt, 2) + val / stl_wid(D, dt), 0), G.nyd(dt) - 1);
+    const long long row = G.base(dt) + (long long)gj * G.nxd(dt) + gi;
+    lo[m] = q < n ? unsigned(row * SP + sys) * unsigned(sizeof(T2)) : kOob;
+  });
+  auto fetch = [&](T2 (&dst)[NL], int L) {
+    // level offsets of the three boxes (uniform)
+    const unsigned lv[3] = {
+        unsigned(min(max(L, 0), G.nzd(0) - 1)) *
+            unsigned((long long)G.nyd(0) * G.nxd(0) * SP * sizeof(T2)),
+        unsigned(min(max(L, 0), G.nzd(1) - 1)) *
+            unsigned((long long)G.nyd(1) * G.nxd(1) * SP * sizeof(T2)),
+        unsigned(min(max(L, 0), G.nzd(2) - 1)) *
+            unsigned((long long)G.nyd(2) * G.nxd(2) * SP * sizeof(T2))};
+    static_for<0, NL>([&](auto Mc) {
+      constexpr int m = decltype(Mc)::value;
+      constexpr int dt = m < NL0 ? 0 : m < NL0 + NL1 ? 1 : 2;
+      dst[m] = buffer_gather_so<T2>(xr, lo[m], lv[dt]);
+    });
+  };
+  auto put = [&](const T2 (&src)[NL], T2* buf) {
+    static_for<0, NL>([&](auto Mc) {
+      constexpr int m = decltype(Mc)::value;
+      constexpr int dt = m < NL0 ? 0 : m < NL0 + NL1 ? 1 : 2;
+      constexpr int n = stl_wid(D, dt) * stl_hgt(D, dt) * SP;
+      const int q = t + kBlock * (m - kFirst[dt]);
+      if (m - kFirst[dt] < (dt == 0 ? NL0 : dt == 1 ? NL1 : NL2) - 1 || q < n)
+        buf[stl_base(D, dt) * SP + q] = src[m];
+    });
+  };
+  // LDS element of column C for this lane: value base(dt) + (wv + dj - jlo) * wid + (g + di -
+  // ilo), system s; with lane = g * SP + s that is lane + wv * wid * SP + a constant
+  const int lane = g * SP + s;
+  const int pw[3] = {lane + wv * stl_wid(D, 0) * SP, lane + wv * stl_wid(D, 1) * SP,
+                     lane + wv * stl_wid(D, 2) * SP};
+  auto lds_at = [&](auto Cc, const T2* buf) {
+    constexpr int C = decltype(Cc)::value;
+    constexpr StencilCol c = kStc[D][C];
+    constexpr int k = (stl_base(D, c.dt) + (c.dj - stl_rng(D, c.dt, 2)) * stl_wid(D, c.dt) +
+                       (c.di - stl_rng(D, c.dt, 0))) * SP;
+    return buf[pw[c.dt] + k];
+  };
+  auto load_coef = [&](double (&dst)[NCR], int k) {
+    const int kk = min(max(k, 0), nzd - 1);
+    const long long r = G.base(D) + ((long long)kk * nyd + jr) * nxd + ir;
+    const unsigned rb = unsigned(r) * unsigned(kStcCoef * 8);
+    static_for<0, NCR>([&](auto Q) {
+      const int slot = Q * SP + s;
+      dst[Q] = buf_f64(cr, slot < kStcCoef ? rb + unsigned(slot) * 8u : kOob);
+    });
+  };
+  constexpr int NBC = 4;
+  constexpr int NB = (NC + NBC - 1) / NBC;
+  double cf[4][NCR];
+  double ar[4], ai[4];
+  T2 xs[2];  // the own column's value at the last two levels (the stored row's x)
+  xs[0].x = xs[0].y = xs[1].x = xs[1].y = T(0);
+  const int L0 = k0 + LO, L1 = k1 - 1 + HI;
+  {
+    T2 first[NL];
+    fetch(first, L0);
+    put(first, pl);  // plane L0 -> buffer 0; the first level's barrier publishes it
+  }
+  load_coef(cf[3], L0 - 1);
+  load_coef(cf[0], L0);
+  load_coef(cf[1], L0 + 1);
+  ar[3] = ai[3] = ar[0] = ai[0] = 0.0;
+  // level L = L0 + n, P = n mod 4; plane L in buffer n mod 2
+  auto level = [&](auto Pc, int L) {
+    constexpr int P = decltype(Pc)::value;
+    const T2* cur = pl + (P % 2) * PS;
+    T2* nxt_buf = pl + ((P + 1) % 2) * PS;
+    T2 nxt[NL];
+    const bool more = L + 1 <= L1;
+    if (more) fetch(nxt, L + 1);
+    load_coef(cf[(P + 2) % 4], L + 2);
+    ar[(P + 1) % 4] = ai[(P + 1) % 4] = 0.0;
+    __syncthreads();  // plane L is in `cur`; every wave is done with level L-1's buffer
+    T2 xb[2][NBC];
+    auto read_batch = [&](T2 (&dst)[NBC], auto Bc) {
+      constexpr int B = decltype(Bc)::value;
+      static_for<0, NBC>([&](auto Jc) {
+        constexpr int C = B * NBC + decltype(Jc)::value;
+        if constexpr (C < NC) dst[decltype(Jc)::value] = lds_at(std::integral_constant<int, C>{}, cur);
+      });
+    };
+    read_batch(xb[0], std::integral_constant<int, 0>{});
+    static_for<0, NB>([&](auto Bc) {
+      constexpr int B = decltype(Bc)::value;
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (B + 1 < NB) read_batch(xb[(B + 1) % 2], std::integral_constant<int, B + 1>{});
+      __builtin_amdgcn_sched_barrier(0);
+      static_for<0, NBC>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        constexpr int C = B * NBC + J;
+        if constexpr (C < NC) {
+          constexpr StencilCol c = kStc[D][C];
+          const double xr_ = double(xb[B % 2][J].x), xi_ = double(xb[B % 2][J].y);
+          const double nwxi = c.dt == D ? -w * xi_ : 0.0, wxr = c.dt == D ? w * xr_ : 0.0;
+          static_for<c.lo, c.hi + 1>([&](auto DK) {
+            constexpr int dk = decltype(DK)::value;
+            constexpr int R = ((P - dk) % 4 + 4) % 4;
+            constexpr int o = stc_slot(D, C, dk);
+            if constexpr (c.dt == D)
+              dfma4_bc<o % SP, (o + 1) % SP>(ar[R], ai[R], cf[R][o / SP], cf[R][(o + 1) / SP],
+                                             xr_, xi_, nwxi, wxr);
+            else
+              dfma2_bc<o % SP>(ar[R], ai[R], cf[R][o / SP], xr_, xi_);
+          });
+        }
+      });
+    });
+    // row k = L - 1 (HI = 1 for every direction) is complete; its own x is the own column's
+    // value at level L - 1, kept from that level
+    static_assert(HI == 1, "rows complete one level after their own");
+    xs[P % 2] = lds_at(std::integral_constant<int, stl_self(D)>{}, cur);
+    const int k = L - HI;
+    constexpr int R = ((P - HI) % 4 + 4) % 4;
+    const bool ok = k >= k0 && k < k1 && row_ok && (run || MODE == 0);
+    const long long tt =
+        (G.base(D) + ((long long)min(max(k, 0), nzd - 1) * nyd + jr) * nxd + ir) * SP + s;
+    stl_store<T, MODE>(ok, unsigned(tt) * unsigned(sizeof(T2)), ar[R], ai[R], xs[(P + 1) % 2],
+                       yr_, br_, ymr, smr, v);
+    if (more) put(nxt, nxt_buf);
+  };
+  for (int L = L0; L <= L1; L += 4) {
+    level(std::integral_constant<int, 0>{}, L);
+    if (L + 1 > L1) break;
+    level(std::integral_constant<int, 1>{}, L + 1);
+    if (L + 2 > L1) break;
+    level(std::integral_constant<int, 2>{}, L + 2);
+    if (L + 3 > L1) break;
+    level(std::integral_constant<int, 3>{}, L + 3);
+  }
+  __syncthreads();  // the last plane's readers are done before the next tile writes buffer 0
+}
+
+template <class T, int SP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* omega,
+                                                     const typename Vec2<T>::type* __restrict__ x,
+                                                     typename Vec2<T>::type* __restrict__ y,
+                                                     const typename Vec2<T>::type* __restrict__ b,
+                                                     typename Vec2<T>::type* __restrict__ ym,
+                                                     typename Vec2<T>::type* __restrict__ sm,
+                                                     const SysScalars* __restrict__ sc,
+                                                     const int* __restrict__ nrun,
+                                                     double* partials) {
+  static_assert(SP == 16, "16-lane rows (the DPP row broadcast)");
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 pl[2 * kStlMaxVal * SP];
+  if (*nrun == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int s = lane % SP, g = lane / SP;
+  const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
+  const double w = omega[s];
+  double v[2] = {0.0, 0.0};
+  const long long vbytes = (long long)A.nb * SP * (long long)sizeof(T2);
+  const __amdgpu_buffer_rsrc_t xr = buf_rsrc(x, vbytes);
+  const __amdgpu_buffer_rsrc_t cr = buf_rsrc(A.coef, (long long)A.nb * kStcCoef * 8);
+  const __amdgpu_buffer_rsrc_t y_r = buf_rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t b_r = buf_rsrc(b, b ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t ym_r = buf_rsrc(ym, ym ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t sm_r = buf_rsrc(sm, sm ? vbytes : 0);
+  // tasks (sweep chunk, band of nbw tile columns, tile row jc, tile column within the band,
+  // direction), direction fastest; one per workgroup in the XCD-aware walk (every wave of a
+  // workgroup takes the same task). The workgroups of one XCD then work on a band nbw tiles
+  // wide, so that the tiles that share an x halo run close in time, within one L2
+  TileWalk tw(A.ntasks);
+  const long long per_kc = (long long)A.njc * A.nic;
+  for (long long task = tw.first; task < tw.last; task += tw.step) {
+    const int d = int(task % 3);
+    const long long q = task / 3;
+    const int kc = int(q / per_kc);
+    const long long r = q % per_kc;
+    const long long full = (long long)A.njc * A.nbw;  // tasks of a full band
+    const int band = int(r / full);
+    const int wb = min(A.nbw, A.nic - band * A.nbw);  // the last band may be narrower
+    const long long rr = r - band * full;
+    const int jc = int(rr / wb);
+    const int ic = band * A.nbw + int(rr % wb);
+    if (d == 0)
+      stl_sweep<T, SP, MODE, 0>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
+                                pl, v);
+    else if (d == 1)
+      stl_sweep<T, SP, MODE, 1>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
+                                pl, v);
+    else
+      stl_sweep<T, SP, MODE, 2>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
+                                pl, v);
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -1935,6 +2251,8 @@ struct mtcocr_handle {
   // counts, and the stencil coefficients of the matrix when every row fits the stencil
   bool grid_order = false;
   GridDims grid;
+  // k_spmv_stl: tile columns per band of its task order (0: one band), per precision and width
+  int stl_band[2][7] = {};
   DevBuf stc;
   bool stc_ok = false;
 
@@ -1994,14 +2312,14 @@ struct mtcocr_handle {
       }
   }
   mtcocr_handle() {
-    // the stencil for 16 systems where the matrix has it, else row blocks for 4..16 systems
-    // where their halos fit, else the row pack (set_caps clamps the layout for 32 and 64);
-    // streaming kernels at about one tile per workgroup (the grid is clamped to the tiles)
-    set_caps(8192, 65536, 2048, 3);
+    // the LDS-staged stencil for 16 systems where the matrix has it, else row blocks for 4..16
+    // systems where their halos fit, else the row pack (set_caps clamps the layout for 32 and
+    // 64); streaming kernels at about one tile per workgroup (the grid is clamped to the tiles)
+    set_caps(8192, 65536, 2048, 4);
     for (auto& row : cap_upd)
       for (int& c : row) c = kMaxBlocks;  // the update pass: one tile per workgroup
     for (auto& row : stc_kc)
-      for (int& c : row) c = 16;
+      for (int& c : row) c = kWholeColumn;  // stencil sweeps over whole columns of the mesh
   }
 
   int64_t device_bytes() const {
@@ -2167,9 +2485,25 @@ StcView stc_view(const mtcocr_handle* h, int kc) {
   return v;
 }
 
+// the same for the LDS-staged kernel: tasks are 4 x kStlNW tiles of rows (one per workgroup)
+inline StcView stl_view(const mtcocr_handle* h, int kc, int band) {
+  StcView v = stc_view<16>(h, kc);
+  v.nic = (h->grid.nx + kStlTI - 1) / kStlTI;
+  v.njc = (h->grid.ny + kStlNW - 1) / kStlNW;
+  v.nbw = band > 0 ? std::min(band, v.nic) : v.nic;
+  const long long nkc = (h->grid.nz + v.kc - 1) / v.kc;
+  v.ntasks = 3LL * nkc * v.njc * v.nic;
+  return v;
+}
+
 template <class TV, int SP>
 int stc_kc_of(const mtcocr_handle* h) {
   return h->stc_kc[std::is_same<TV, float>::value ? 1 : 0][mtcocr_handle::sp_index(SP)];
+}
+
+template <class TV, int SP>
+int stl_band_of(const mtcocr_handle* h) {
+  return h->stl_band[std::is_same<TV, float>::value ? 1 : 0][mtcocr_handle::sp_index(SP)];
 }
 
 // The row kernels address x with 32-bit byte offsets: batch vectors of nb x SP values of
@@ -2194,8 +2528,14 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                                                           omega, x, y, b, ym, sm, sc, nrun, part);
         return;
       }
+      if (layout == 4 && h->stc_ok) {
+        k_spmv_stl<TV, SP, MODE><<<grid, kBlock, 0, st>>>(
+            stl_view(h, stc_kc_of<TV, SP>(h), stl_band_of<TV, SP>(h)), omega, x, y, b, ym, sm,
+            sc, nrun, part);
+        return;
+      }
     }
-    if (layout == 3) layout = 2;  // the stencil layout serves 16-lane rows only
+    if (layout >= 3) layout = 2;  // the stencil layouts serve 16-lane rows only
     const int lds = blk_lds_bytes<TV>(h);
     if (layout == 2 && lds > 0) {
       k_spmv_blk<TV, SP, MODE><<<grid, kBlock, lds, st>>>(blk_view(h), h->halo_max, omega, x, y,
@@ -3133,7 +3473,9 @@ int mtcocr_pcocr_2x2_folded(int n, const int* ig, const int* jg, const int* idi,
   int rc = mtcocr_create(dev, &h);
   if (rc) return rc;
   // locality order from the mesh when it is a structured hexahedral one (else the reference's)
-  (void)mtcocr_set_mesh_order(h, kpar, n_nodes_c, nver, n_edges_c, edges, 16);
+  if (mtcocr_set_mesh_order(h, kpar, n_nodes_c, nver, n_edges_c, edges, MTCOCR_ORDER_GRID) !=
+      MTCOCR_OK)
+    (void)mtcocr_set_mesh_order(h, kpar, n_nodes_c, nver, n_edges_c, edges, 16);
   rc = mtcocr_set_matrix_block2x2(h, n / 2, ig, jg, idi, ijg, di, gg);
   if (!rc)
     rc = mtcocr_set_gradient_mesh(h, kpar, n_edges_c, n_nodes_c, nvkat, nver, sigma3d, edges,
@@ -3285,7 +3627,7 @@ int mtcocr_set_launch(mtcocr_handle* h, int cap_spmv, int cap_vec, int cap_node,
   if (!h) return MTCOCR_ERR_ARG;
   auto bad = [](int c) { return c != 0 && (c < kXcd || c > kMaxBlocks || c % kXcd); };
   if (bad(cap_spmv) || bad(cap_vec) || bad(cap_node)) return MTCOCR_ERR_ARG;
-  if (spmv_layout < -1 || spmv_layout > 3) return MTCOCR_ERR_ARG;
+  if (spmv_layout < -1 || spmv_layout > 4) return MTCOCR_ERR_ARG;
   h->set_caps(cap_spmv, cap_vec, cap_node, spmv_layout);
   drop_graph(h);  // the captured iterations hold the old grids
   return 0;
@@ -3298,7 +3640,7 @@ namespace {
 // the row pack where they do not apply)
 int effective_layout(const mtcocr_handle* h, int q, int k) {
   int l = h->spmv_layout[q][k];
-  if (l == 3 && !(h->stc_ok && k == 4)) l = 2;
+  if (l >= 3 && !(h->stc_ok && k == 4)) l = 2;
   if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
   if (!rows_fit(h, 1 << k, q == 0 ? 16 : 8)) l = 0;
   return l;
@@ -3315,10 +3657,26 @@ int mtcocr_set_stencil_sweep(mtcocr_handle* h, int kc) {
   return 0;
 }
 
+int mtcocr_set_stencil_band(mtcocr_handle* h, int band) {
+  if (!h || band < 0) return MTCOCR_ERR_ARG;
+  for (auto& row : h->stl_band)
+    for (int& c : row) c = band;
+  drop_graph(h);  // captured launches hold the old task order
+  return 0;
+}
+
+int mtcocr_get_stencil_band(mtcocr_handle* h, int precision, int* band) {
+  if (!h || !band) return MTCOCR_ERR_ARG;
+  if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
+  *band = h->stl_band[precision][mtcocr_handle::sp_index(h->nsys > 0 ? h->sp : 1)];
+  return 0;
+}
+
 int mtcocr_get_stencil_sweep(mtcocr_handle* h, int precision, int* kc) {
   if (!h || !kc) return MTCOCR_ERR_ARG;
   if (precision != MTCOCR_PREC_FP64 && precision != MTCOCR_PREC_MIXED) return MTCOCR_ERR_ARG;
   *kc = h->stc_kc[precision][mtcocr_handle::sp_index(h->nsys > 0 ? h->sp : 1)];
+  if (h->grid_order && h->grid.nz > 0) *kc = std::min(*kc, h->grid.nz);  // whole columns
   return 0;
 }
 
@@ -3395,19 +3753,29 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         if (fam == 0 && SP >= 4 && rows_fit(h, SP, int(sizeof(T2)))) {
           layouts.push_back(1);
           if (SP <= 16 && blk_lds_bytes<T>(h) > 0) layouts.push_back(2);
-          if (SP == 16 && h->stc_ok) layouts.push_back(3);
+          if (SP == 16 && h->stc_ok) {
+            layouts.push_back(3);
+            layouts.push_back(4);
+          }
         }
         int best_kc = h->stc_kc[q][ki];
+        int best_band = h->stl_band[q][ki];
         for (int layout : layouts) {
           // the stencil kernel's sweep length: a few lengths, each at every cap
           std::vector<int> kcs{0};
-          if (layout == 3) {
+          if (layout >= 3) {
             kcs = {2, 4, 8, 16, h->grid.nz};
+            if (layout == 4) kcs = {8, 16, (h->grid.nz + 1) / 2, h->grid.nz};
             std::sort(kcs.begin(), kcs.end());
             kcs.erase(std::unique(kcs.begin(), kcs.end()), kcs.end());
           }
+          // the LDS-staged kernel's task order: bands of tile columns (0: one band)
+          std::vector<int> bands{h->stl_band[q][ki]};
+          if (layout == 4) bands = {0, 5, 9, 13};
           for (int kc : kcs) {
-          if (layout == 3) h->stc_kc[q][ki] = kc;
+          for (int band : bands) {
+          if (layout >= 3) h->stc_kc[q][ki] = kc;
+          if (layout == 4) h->stl_band[q][ki] = band;
           // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
           // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
           // twice that
@@ -3416,6 +3784,10 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
             if constexpr (SP == 16)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_stc<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0 && layout == 4) {
+            if constexpr (SP == 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_stl<T, SP, 0>), kBlock, 0));
           } else if (fam == 0 && layout == 2) {
             if constexpr (SP >= 4 && SP <= 16)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -3499,8 +3871,10 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
               best_ms[fam] = avg;
               best[fam] = cap;
               if (fam == 0) best_layout = layout;
-              if (fam == 0 && layout == 3) best_kc = kc;
+              if (fam == 0 && layout >= 3) best_kc = kc;
+              if (fam == 0 && layout == 4) best_band = band;
             }
+          }
           }
           }
         }
@@ -3508,6 +3882,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         if (fam == 0) {
           h->spmv_layout[q][ki] = best_layout;
           h->stc_kc[q][ki] = best_kc;
+          h->stl_band[q][ki] = best_band;
         }
       }
       return 0;

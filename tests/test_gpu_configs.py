@@ -10,28 +10,20 @@ the GPU through the C-ABI:
             frequencies from 1e-2 to 1e1 Hz x 2 polarizations: one batch of 8, mixed and FP64.
 
 Residual bounds. The FP64 solver stops, as the reference does, on the MRS-smoothed residual; the
-raw iterate it returns sits above eps by an amount that rounding decides. The oracle's own
-returned x reaches 1.06e-7 on the full-size benchmark model (tests/golden/config2_oracle.json
-and its README) and, on system 19 of config 3's 48^3 model (3.98 Hz, Y), 1.65e-7 on one thread
-and 5.5e-8 on eight, where the GPU's (the same 683 iterations) reaches 2.9e-7. FP64 answers are
-therefore held to a true relative residual of 5e-7, a bound on the method's attainable accuracy
-at the stop point rather than on eps, next to the stop test itself (MRS residual < eps) and the
-impedances. The mixed solver's refinement tests the true FP64 residual itself, so its answers
-are held to eps = 1e-8.
+raw iterate it returns sits above eps by an amount that rounding decides (conftest's
+FP64_TRUE_BOUND, 5e-7, and its measurements: on system 19 of the 48^3 model here the oracle's
+own x reaches 1.65e-7 on one thread and 5.5e-8 on eight, the GPU's 2.9e-7 after the same 683
+iterations). FP64 answers are held to it next to the stop test itself (MRS residual < eps)
+and the impedances. The mixed solver's refinement tests the true FP64 residual itself, so its
+answers are held to eps = 1e-8.
 """
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import gpu_available
+from conftest import FP64_TRUE_BOUND, gpu_available, say
 
 pytestmark = pytest.mark.gpu
-
-FP64_TRUE_BOUND = 5e-7
-
-
-def say(msg):
-    print(f"[gpu-test] {msg}", flush=True)
 
 
 def ks_operator(p, kd, kg, sd, sg):
@@ -61,6 +53,7 @@ def solve_model(M, m, freqs, precisions, order=-1):
     prs = np.stack([M.assemble_rhs(m, f, a) for f, a in cases])
     om = np.array([M.omega_of(f) for f, _ in cases])
     out = {}
+    say(f"  {m.name}: assembled, solving")
     with Solver(0) as S:
         S.set_mesh_order(ma, order)
         S.set_matrix_ks(p, kd, kg, sd, sg)
@@ -68,6 +61,7 @@ def solve_model(M, m, freqs, precisions, order=-1):
         for prec in precisions:
             x, st = S.solve_batch(om, prs, eps=1e-8, maxiter=40000, precision=prec)
             out[prec] = (x, st)
+            say(f"  {m.name}: {prec} batch done ({st[0]['seconds']:.1f} s)")
         info = S.info()
     return cases, prs, om, ks_operator(p, kd, kg, sd, sg), out, ma, info
 
@@ -173,6 +167,7 @@ def test_config5_large_batch(M):
     reported against the 288 GB of one MI355X."""
     if not gpu_available():
         pytest.skip("no GPU visible")
+    say("config 5: building the 256 x 256 x 128 model")
     m = M.ellipsoid_model(256, 256, 128, seed=0)
     freqs = M.frequencies(1e-2, 1e1, 4)
     say(f"config 5: nb = {m.n_edges}, 8 systems")
