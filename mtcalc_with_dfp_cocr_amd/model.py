@@ -209,6 +209,196 @@ def ellipsoid_model(ncx: int, ncy: int, ncz: int, seed: int, n_ellipsoids: int =
                          name=f"ellip{ncx}x{ncy}x{ncz}_s{seed}"))
 
 
+# ------------------------------------------------------------------------------------------
+# the regular mesh builder's axes and box objects (the reference's example input format)
+
+
+@dataclass
+class MeshSettings:
+    """The first four lines of the mesh builder's settings.cfg (RegularMeshBuilder/main.cpp:
+    175-190, 1253-1264): start step, sparse (growth) factor, gap between the receivers' box and
+    the sparse part, and distance from the receivers' box to the domain boundary, each for XY
+    (both horizontal directions) and Z; eps decides when an inserted line replaces a near one."""
+
+    step_xy: float
+    step_z: float
+    sparse_xy: float
+    sparse_z: float
+    gap_xy: float
+    gap_z: float
+    far_xy: float
+    far_z: float
+    eps: float = 0.1
+
+    def per_axis(self):
+        return [(self.step_xy, self.sparse_xy, self.gap_xy, self.far_xy)] * 2 + \
+               [(self.step_z, self.sparse_z, self.gap_z, self.far_z)]
+
+
+def read_settings(path) -> MeshSettings:
+    """Four pairs of numbers at the top of settings.cfg (main.cpp:1253-1264); the rest of the
+    file (SLAE eps, iteration cap, notes) is not part of the mesh."""
+    v = [float(s) for s in Path(path).read_text().split()[:8]]
+    return MeshSettings(*v)
+
+
+def read_objects(path) -> np.ndarray:
+    """objects: a count, then x0 x1 y0 y1 z0 z1 sigma per box (main.cpp:1116-1145)."""
+    tok = Path(path).read_text().split()
+    n = int(tok[0])
+    return np.array([float(s) for s in tok[1:1 + 7 * n]]).reshape(n, 7)
+
+
+def read_layers(path) -> np.ndarray:
+    """z_sig_2d: a count, then (z, sigma) per layer, deepest first (main.cpp:1169-1202)."""
+    tok = Path(path).read_text().split()
+    n = int(tok[0])
+    return np.array([float(s) for s in tok[1:1 + 2 * n]]).reshape(n, 2)
+
+
+def read_receivers(path) -> np.ndarray:
+    """xyzVectorB: a count, then x y z per receiver (main.cpp:1219-1240)."""
+    tok = Path(path).read_text().split()
+    n = int(tok[0])
+    return np.array([float(s) for s in tok[1:1 + 3 * n]]).reshape(n, 3)
+
+
+def _nearest_line(axis: list, value: float) -> int:
+    """Index of the line nearest to value by bisection, the upper one on a tie; the end lines
+    for values outside (BinarySearchInVectorSorted, main.cpp:241-265)."""
+    if value < axis[0]:
+        return 0
+    if value > axis[-1]:
+        return len(axis) - 1
+    lo, hi = 0, len(axis) - 1
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if value > axis[mid]:
+            lo = mid
+        else:
+            hi = mid
+    return lo if abs(axis[lo] - value) < abs(axis[hi] - value) else hi
+
+
+def _insert_line(axis: list, value: float, eps: float) -> None:
+    """A line at `value` (InsertCoordinateIntoSortedArray, main.cpp:487-512): the nearest line
+    moves there when it lies within eps of the local step, otherwise a line is added; values
+    outside the axis are ignored."""
+    if value < axis[0] or value > axis[-1]:
+        return
+    k = _nearest_line(axis, value)
+    nbr = k - 1 if (axis[k] > value and k != 0) or k == len(axis) - 1 else k + 1
+    step = abs(axis[k] - axis[nbr])
+    if abs(axis[k] - value) / step > eps:
+        axis.insert(k if axis[k] > value else k + 1, value)
+    else:
+        axis[k] = value
+
+
+def _uniform(step: float, lo: float, hi: float) -> list:
+    """Build1DUniformMesh (main.cpp:268-284): the step shrunk to divide [lo, hi] evenly."""
+    n = int((hi - lo) / step + 0.5)
+    if abs(step * n - (hi - lo)) > 1e-5:
+        n += 1
+    h = (hi - lo) / n
+    return [lo + i * h for i in range(n + 1)]
+
+
+def _sparse(step: float, growth: float, lo: float, hi: float, outward_up: bool) -> list:
+    """Build1DSparsedMesh (main.cpp:296-340): steps growing by `growth` away from the uniform
+    part, until the boundary is passed."""
+    out = []
+    if outward_up:
+        c = lo
+        out.append(c)
+        while True:
+            c += step
+            step *= growth
+            out.append(c)
+            if not c < hi:
+                break
+        return out
+    c = hi
+    out.append(c)
+    while True:
+        c -= step
+        step *= growth
+        out.append(c)
+        if not c > lo:
+            break
+    return out[::-1]
+
+
+def _round_down(value: float, dec: float) -> float:
+    """RoundDouble_ (main.cpp:572-576): the multiple of dec at or below a positive value, one
+    step below a negative one."""
+    n = int(abs(value / dec))
+    return -(n + 1) * dec if value < 0 else n * dec
+
+
+def builder_axis(r_lo: float, r_hi: float, step: float, growth: float, gap: float,
+                 far: float) -> list:
+    """One axis of the regular mesh builder (CalculateModelBounds, BuildBaseMesh1D and
+    AlignMeshBounds, main.cpp:344-376,441-462,579-683): uniform steps over the receivers' extent
+    widened by `gap` (rounded to whole steps), growing steps out to `far` beyond the receivers'
+    extent, the outer lines placed on the boundary and a last step under 0.3 of its neighbour
+    merged into it."""
+    lo, hi = r_lo - gap, r_hi + gap
+    n = int((hi - lo) / step + 0.5)
+    if abs(n * step - (hi - lo)) > 1e-7:
+        n += 1
+        lo = _round_down(lo, step)
+        hi = lo + n * step
+    b = [r_lo - far, lo, hi, r_hi + far]
+    mid = _uniform(step, b[1], b[2])
+    left = _sparse(step, growth, b[0], b[1], False)
+    right = _sparse(step, growth, b[2], b[3], True)
+    ax = left[:-1] + mid[:-1] + right
+    ax[0] = b[0]
+    if ax[1] - ax[0] < (ax[2] - ax[1]) * 0.3:
+        del ax[1]
+    ax[-1] = b[3]
+    if ax[-1] - ax[-2] < (ax[-2] - ax[-3]) * 0.3:
+        del ax[-2]
+    return ax
+
+
+def box_model(receivers: np.ndarray, settings: MeshSettings, objects: np.ndarray,
+              layers: np.ndarray, name: str = "boxes") -> Model:
+    """A model in the reference's example input format (BuildMesh, main.cpp:1086-1113): the
+    builder's axes around the receivers, each box's six faces and each layer's top inserted as
+    lines, a layered earth (z_sig_2d, deepest first) under air at 1e-8 S/m, boxes painted over
+    it in file order. sigma0 is the layered earth (SetNormalMaterials, main.cpp:750-757).
+
+    Layer lines: z_sig_2d's layer tops. (The reference derives each layer's lower line from a
+    layer top it has not read yet, main.cpp:1193; for a file whose layers share one
+    conductivity, as the example's do, only the mesh lines depend on that.)"""
+    rec = np.asarray(receivers, dtype=np.float64)
+    axes = []
+    for d, (step, growth, gap, far) in enumerate(settings.per_axis()):
+        axes.append(builder_axis(rec[:, d].min(), rec[:, d].max(), step, growth, gap, far))
+    for o in objects:
+        for f in range(6):
+            _insert_line(axes[f // 2], float(o[f]), settings.eps)
+    lay = np.asarray(layers, dtype=np.float64)
+    for ztop in lay[:, 0]:
+        _insert_line(axes[2], float(ztop), settings.eps)
+    x, y, z = (np.array(a) for a in axes)
+    tops = lay[:, 0].copy()
+    tops[-1] = 0.0
+    sig_layers = lay[:, 1].copy()
+    xc, yc, zc = Model(x, y, z, np.zeros(1), np.zeros(1), tops, sig_layers).cell_centres()
+    sigma0 = _layered(zc, tops, sig_layers)
+    sigma0[zc > 0] = AIR_SIGMA
+    sigma = sigma0.copy().reshape(len(z) - 1, len(y) - 1, len(x) - 1)
+    for o in objects:
+        ix = [_nearest_line(axes[0], o[0]), _nearest_line(axes[0], o[1])]
+        iy = [_nearest_line(axes[1], o[2]), _nearest_line(axes[1], o[3])]
+        iz = [_nearest_line(axes[2], o[4]), _nearest_line(axes[2], o[5])]
+        sigma[iz[0]:iz[1], iy[0]:iy[1], ix[0]:ix[1]] = o[6]
+    return _finish(Model(x, y, z, sigma.ravel(), sigma0, tops, sig_layers, rec, name=name))
+
+
 def _finish(m: Model) -> Model:
     m.sigma = round_sig(m.sigma)
     m.sigma0 = round_sig(m.sigma0)

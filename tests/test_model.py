@@ -329,3 +329,66 @@ def test_files_roundtrip(M, tiny_model, tmp_path):
     assert (d / "kuslau").read_text().split() == [str(2 * s.nb), "1.000000e-08", "10000"]
     mats = np.loadtxt(d / "sig3d")
     assert mats[0, 1] == 1e-8 and int(mats[-1, 0]) == len(mats)
+
+
+# ---- the mesh builder's axes on the reference's example ------------------------------------
+
+def _example():
+    import json
+    from pathlib import Path
+
+    return json.loads((Path(__file__).resolve().parent / "golden" / "dtm1_example.json").read_text())
+
+
+def test_builder_reproduces_the_example_mesh(M):
+    """The reference's example (CalculationExample: settings.cfg, objects, z_sig_2d, xyzVectorB)
+    through the restated builder gives the mesh the reference built from it: 135,152 edges,
+    14,940 Dirichlet (boundary) edges and 38,896 interior nodes, the sizes measured on the
+    compiled reference's run of that example. Every box face is a mesh line, the boxes' cells
+    carry their conductivity, the rest of the earth the half-space's, the air 1e-8."""
+    d = _example()
+    s = M.MeshSettings(*d["settings"])
+    obj = np.array(d["objects"])
+    m = M.box_model(np.array(d["receivers"]), s, obj, np.array(d["layers"]))
+    assert m.n_edges == 135152
+    ma = M.mesh_arrays(m)
+    assert int((ma.is_node_bound == 0).sum()) == 38896
+    p, kd, kg, sd, sg = M.assemble_ks(m)
+    assert int((kd == 1.0).sum()) == 14940
+    axes = (m.x, m.y, m.z)
+    for o in obj:
+        for f in range(6):
+            assert np.min(np.abs(axes[f // 2] - o[f])) == 0.0
+    xc, yc, zc = m.cell_centres()
+    expect = np.where(zc > 0, 1e-8, 0.01)
+    for o in obj:
+        inside = ((xc > o[0]) & (xc < o[1]) & (yc > o[2]) & (yc < o[3]) & (zc > o[4])
+                  & (zc < o[5]))
+        expect[inside] = o[6]
+    np.testing.assert_array_equal(m.sigma, M.round_sig(expect))
+    assert np.all(m.sigma0[zc < 0] == 0.01) and np.all(m.sigma0[zc > 0] == 1e-8)
+
+
+def test_builder_axis_rules(M):
+    """One axis by the builder's rules: uniform steps over the receivers' extent plus the gap,
+    rounded out to whole steps; steps growing by the sparse factor outside; the end lines on the
+    boundary at `far` beyond the receivers; a last step under 0.3 of its neighbour merged."""
+    ax = np.array(M.builder_axis(-25000.0, 25000.0, 625.0, 1.5, 500.0, 65000.0))
+    assert ax[0] == -90000.0 and ax[-1] == 90000.0
+    i0 = int(np.flatnonzero(ax == -25625.0)[0])
+    i1 = int(np.flatnonzero(ax == 26250.0)[0])
+    h = np.diff(ax)
+    assert i1 - i0 == 83 and np.allclose(h[i0:i1], 625.0)
+    right = h[i1:-1]   # outward from the uniform part, the aligned last step aside
+    left = h[1:i0][::-1]
+    np.testing.assert_allclose(right, 625.0 * 1.5 ** np.arange(len(right)))
+    np.testing.assert_allclose(left, 625.0 * 1.5 ** np.arange(len(left)))
+    assert h[-1] >= 0.3 * h[-2] and h[0] >= 0.3 * h[1]
+    # a line within eps of the local step moves; farther away one is added
+    a = [0.0, 1.0, 2.0, 3.0]
+    M._insert_line(a, 1.05, 0.1)
+    assert a == [0.0, 1.05, 2.0, 3.0]
+    M._insert_line(a, 2.5, 0.1)
+    assert a == [0.0, 1.05, 2.0, 2.5, 3.0]
+    M._insert_line(a, 7.0, 0.1)
+    assert a == [0.0, 1.05, 2.0, 2.5, 3.0]
