@@ -116,6 +116,60 @@ def collect(root, m: M.Model, freqs, receivers) -> np.ndarray:
     return np.stack(out)
 
 
+def example_model(folder, step_xy: float | None = None, step_z: float | None = None):
+    """The model of a reference calculation folder (CalculationExample: settings.cfg, objects,
+    z_sig_2d, xyzVectorB, frequencies; read as RegularMeshBuilder and CalcStarter read them,
+    RegularMeshBuilder/main.cpp:1116-1264, CalcStarter/main.cpp:134-176), through the restated
+    mesh builder. step_xy / step_z override the settings' start steps. Returns the model, the
+    frequencies (Hz, file order) and the receivers (R, 3)."""
+    folder = Path(folder)
+    s = M.read_settings(folder / "settings.cfg")
+    if step_xy:
+        s.step_xy = float(step_xy)
+    if step_z:
+        s.step_z = float(step_z)
+    tok = (folder / "frequencies").read_text().split()
+    freqs = [float(v) for v in tok[1:1 + int(tok[0])]]
+    rec = M.read_receivers(folder / "xyzVectorB")
+    m = M.box_model(rec, s, M.read_objects(folder / "objects"), M.read_layers(folder / "z_sig_2d"),
+                    name=folder.name or "example")
+    return m, freqs, rec
+
+
+def read_solver_settings(path) -> tuple[float, int]:
+    """The SLAE tolerance and iteration cap that follow the mesh numbers in settings.cfg
+    (CalcStarter/main.cpp:134-149); RegularMeshBuilder passes them on through issc.txt to
+    BuildMatrix, which writes them into kuslau (RegularMeshBuilder/main.cpp:2137,
+    BuildMatrix/t_global_slae.cpp:174-190)."""
+    tok = Path(path).read_text().split()
+    return float(tok[8]), int(tok[9])
+
+
+RZ_COLUMNS = ("northing (m)", "easting (m)", "periods (s)", "Zxx (Ohm)", "Zxy (Ohm)", "Zyx (Ohm)",
+              "Zyy (Ohm)", "Rxx (Ohm)", "Rxy (Ohm)", "Ryx (Ohm)", "Ryy (Ohm)")
+
+
+def write_rz_table(path, freqs, receivers, Z) -> None:
+    """The impedance table in the layout of the reference's published result
+    (CalculationResults/625_500_reg_R_Z.txt): a header line, then per receiver (in the given
+    order) and period (ascending) the receiver's northing and easting, the period, |Z| of the
+    four components and the four apparent resistivities |Z|^2/(omega mu0). The table is in a
+    (north, east, down) frame: northing = the model's x, easting = the model's -y."""
+    rec = np.atleast_2d(np.asarray(receivers, dtype=np.float64))
+    freqs = np.asarray(freqs, dtype=np.float64)
+    order = np.argsort(1.0 / freqs, kind="stable")
+    lines = ["\t".join(RZ_COLUMNS)]
+    for r in range(len(rec)):
+        for k in order:
+            z = np.abs(Z[k, r])
+            rho = z ** 2 / (M.omega_of(float(freqs[k])) * M.MU_0)
+            vals = [rec[r, 0], -rec[r, 1] + 0.0, 1.0 / freqs[k], z[0, 0], z[0, 1], z[1, 0], z[1, 1],
+                    rho[0, 0], rho[0, 1], rho[1, 0], rho[1, 1]]
+            lines.append("\t".join(f"{v:.6e}" for v in vals))
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
 def main(argv=None) -> int:
     import argparse
 
@@ -123,27 +177,45 @@ def main(argv=None) -> int:
                                  "working directories, the COCR_FP solver, impedance table")
     ap.add_argument("root", help="calculation root (Calculation/ and Results/ are created here)")
     ap.add_argument("--model", choices=("config1", "random"), default="config1")
+    ap.add_argument("--example", metavar="DIR",
+                    help="a reference calculation folder (settings.cfg, objects, z_sig_2d, "
+                         "xyzVectorB, frequencies) instead of --model; also writes "
+                         "Results/R_Z.txt in the reference's published table layout")
+    ap.add_argument("--step-xy", type=float, default=None,
+                    help="with --example: horizontal start step overriding settings.cfg")
     ap.add_argument("--cells", type=int, nargs=3, default=(32, 32, 24),
                     help="random model: cells in x, y, z")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--freqs", type=float, nargs="+", default=[0.1, 1.0, 10.0])
-    ap.add_argument("--eps", type=float, default=1e-8)
-    ap.add_argument("--maxiter", type=int, default=10000)
+    ap.add_argument("--eps", type=float, default=None,
+                    help="SLAE tolerance (default: settings.cfg's with --example, else 1e-8)")
+    ap.add_argument("--maxiter", type=int, default=None)
     ap.add_argument("--per-directory", action="store_true",
                     help="one solver process per directory (the reference's way)")
     ap.add_argument("--strict", action="store_true")
     a = ap.parse_args(argv)
-    if a.model == "config1":
-        m = M.config1_model()
+    eps, maxiter = 1e-8, 10000
+    if a.example:
+        m, freqs, rec = example_model(a.example, step_xy=a.step_xy)
+        eps, maxiter = read_solver_settings(Path(a.example) / "settings.cfg")
     else:
-        m = M.random_model(*a.cells, seed=a.seed, h=50.0, name="random")
-    dirs = prepare(a.root, m, a.freqs, a.eps, a.maxiter)
+        if a.model == "config1":
+            m = M.config1_model()
+        else:
+            m = M.random_model(*a.cells, seed=a.seed, h=50.0, name="random")
+        freqs = a.freqs
+        g = np.arange(-200.0, 201.0, 100.0)
+        rx, ry = np.meshgrid(g, g, indexing="ij")
+        rec = np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1)
+    eps = a.eps if a.eps is not None else eps
+    maxiter = a.maxiter if a.maxiter is not None else maxiter
+    dirs = prepare(a.root, m, freqs, eps, maxiter)
     rc = run(dirs, batch=not a.per_directory, strict=a.strict)
     if rc != 0:
         return rc
-    g = np.arange(-200.0, 201.0, 100.0)
-    rx, ry = np.meshgrid(g, g, indexing="ij")
-    collect(a.root, m, a.freqs, np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1))
+    Z = collect(a.root, m, freqs, rec)
+    if a.example:
+        write_rz_table(Path(a.root) / "Results" / "R_Z.txt", freqs, rec, Z)
     print(f"results in {Path(a.root) / 'Results'}")
     return 0
 
