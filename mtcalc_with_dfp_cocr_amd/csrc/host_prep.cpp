@@ -400,6 +400,56 @@ int build_gradient(int kpar, int n_edges, int n_nodes_c, const int* nvkat, const
   return 0;
 }
 
+bool gradient_on_grid(const GradientEll& G, const GridDims& d) {
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  if (nx < 3 || ny < 3 || nz < 3) return false;
+  const int64_t m = (int64_t)(nx - 2) * (ny - 2) * (nz - 2);
+  if (G.m != m || G.n != d.base(2) + d.count(2) || G.wg != 6 || G.wgt != 2) return false;
+  auto row = [&](int dir, int i, int j, int k) -> int64_t {
+    return d.base(dir) + ((int64_t)k * d.nyd(dir) + j) * d.nxd(dir) + i;
+  };
+  auto node = [&](int i, int j, int k) -> int64_t {
+    if (i < 1 || j < 1 || k < 1 || i > nx - 2 || j > ny - 2 || k > nz - 2) return -1;
+    return (i - 1) + (int64_t)(nx - 2) * ((j - 1) + (int64_t)(ny - 2) * (k - 1));
+  };
+  for (int k = 1; k <= nz - 2; ++k)
+    for (int j = 1; j <= ny - 2; ++j)
+      for (int i = 1; i <= nx - 2; ++i) {
+        const int64_t n = node(i, j, k);
+        const int64_t e[6] = {row(2, i, j, k - 1), row(1, i, j - 1, k), row(0, i - 1, j, k),
+                              row(0, i, j, k),     row(1, i, j, k),     row(2, i, j, k)};
+        for (int q = 0; q < 6; ++q) {
+          if (G.gcol[(size_t)q * m + n] != e[q]) return false;
+          if (G.gval[(size_t)q * m + n] != (q < 3 ? 0.5 : -0.5)) return false;
+        }
+      }
+  const int64_t nb = G.n;
+  for (int dir = 0; dir < 3; ++dir)
+    for (int k = 0; k < d.nzd(dir); ++k)
+      for (int j = 0; j < d.nyd(dir); ++j)
+        for (int i = 0; i < d.nxd(dir); ++i) {
+          const int64_t e = row(dir, i, j, k);
+          const int64_t lo = node(i, j, k);
+          const int64_t up = node(i + (dir == 0), j + (dir == 1), k + (dir == 2));
+          int64_t want_c[2] = {0, 0};
+          double want_v[2] = {0.0, 0.0};
+          int c = 0;
+          if (lo >= 0) {
+            want_c[c] = lo;
+            want_v[c++] = -0.5;
+          }
+          if (up >= 0) {
+            want_c[c] = up;
+            want_v[c++] = 0.5;
+          }
+          for (int q = 0; q < 2; ++q) {
+            if (G.gtval[(size_t)q * nb + e] != want_v[q]) return false;
+            if (q < c && G.gtcol[(size_t)q * nb + e] != want_c[q]) return false;
+          }
+        }
+  return true;
+}
+
 namespace {
 
 constexpr int kUnset = std::numeric_limits<int>::min();

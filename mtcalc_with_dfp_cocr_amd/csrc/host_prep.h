@@ -143,6 +143,14 @@ struct GridDims {
 int grid_order_from_mesh(int kpar, int n_nodes, const int* nver, int n_edges, const int* edges,
                          std::vector<int>& order, GridDims& dims);
 
+// True when G (held with its edges in the grid order of `dims`) is the gradient of the whole box:
+// one row per interior node in (k, j, i) order, each with the node's six edges in ascending
+// reference edge order (z below, y below, x below, x above, y above, z above) and values
+// +1/2, +1/2, +1/2, -1/2, -1/2, -1/2; each G^T row the edge's lower then upper node (-1/2,
+// +1/2), the interior ones only. The fused grid preconditioner kernel computes this structure
+// instead of reading it.
+bool gradient_on_grid(const GradientEll& G, const GridDims& dims);
+
 // Stencil coefficients (nb x kStcCoef, row-major) of a matrix held in the grid order (its row
 // pack and diagonal in `E`). Fails (negative code) when a row has an entry outside its stencil
 // or a complex entry in a cross position; the layout then does not apply.

@@ -1533,6 +1533,31 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* om
   block_partials<SP, 2>(v, partials);
 }
 
+// The folded preconditioner's arithmetic, shared by the node and edge kernels and the fused
+// grid kernel so that every path gives the same bits: G and G^T products accumulate with fused
+// multiply-adds in entry order, complex products round each term.
+template <class T>
+__device__ inline T fmad(T a, T b, T c);
+template <>
+__device__ inline double fmad<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ inline float fmad<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+template <class T, class T2>
+__device__ inline void pc_acc(T& yr, T& yi, T g, T2 v) {
+  yr = fmad<T>(g, v.x, yr);
+  yi = fmad<T>(g, v.y, yi);
+}
+template <class T, class T2>
+__device__ inline void pc_cmul(T ar, T ai, T2 d, T& yr, T& yi) {
+#pragma clang fp contract(off)
+  yr = ar * d.x - ai * d.y;
+  yi = ar * d.y + ai * d.x;
+}
+
 // help = D_N^-1 (G v), v = z + beta*a (USE_BETA) or v = z  (FoldedPreconditioner.cpp:598-615)
 template <class T, int SP, bool USE_BETA>
 __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
@@ -1581,8 +1606,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
       if (k < G.wg) {
         T2 vv = zv[k];
         if (USE_BETA) vv = caxpy(vv, av[k], br, bi);
-        hr += gv[k] * vv.x;
-        hi += gv[k] * vv.y;
+        pc_acc(hr, hi, gv[k], vv);
       }
     }
     for (int k = kGU; k < G.wg; ++k) {
@@ -1590,13 +1614,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_node(GView<T> G,
       const T g = G.gval[(size_t)k * G.m + i];
       T2 vv = z[(size_t)e * SP + s];
       if (USE_BETA) vv = caxpy(vv, a[(size_t)e * SP + s], br, bi);
-      hr += g * vv.x;
-      hi += g * vv.y;
+      pc_acc(hr, hi, g, vv);
     }
-    const T2 d = dn[t];
     T2 o;
-    o.x = hr * d.x - hi * d.y;
-    o.y = hi * d.x + hr * d.y;
+    pc_cmul(hr, hi, dn[t], o.x, o.y);
     help[t] = o;
   }
 }
@@ -1620,18 +1641,12 @@ __device__ inline void gt_apply(const GView<T>& G, const typename Vec2<T>::type*
   for (int k = 0; k < kGU; ++k)
     if (k < G.wgt) hv[k] = help[(size_t)nc[k] * SP + s];
 #pragma unroll
-  for (int k = 0; k < kGU; ++k) {
-    if (k < G.wgt) {
-      yr += gv[k] * hv[k].x;
-      yi += gv[k] * hv[k].y;
-    }
-  }
+  for (int k = 0; k < kGU; ++k)
+    if (k < G.wgt) pc_acc(yr, yi, gv[k], hv[k]);
   for (int k = kGU; k < G.wgt; ++k) {
     const int node = G.gtcol[(size_t)k * G.n + e];
     const T g = G.gtval[(size_t)k * G.n + e];
-    const T2 h = help[(size_t)node * SP + s];
-    yr += g * h.x;
-    yi += g * h.y;
+    pc_acc(yr, yi, g, help[(size_t)node * SP + s]);
   }
 }
 
@@ -1652,9 +1667,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_plain(
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
     const int e = int(t / SP);
-    const T2 xv = v[t], d = diag_inv<T>(da, e, s);
-    T yr = xv.x * d.x - xv.y * d.y;
-    T yi = xv.x * d.y + xv.y * d.x;
+    const T2 xv = v[t];
+    T yr, yi;
+    pc_cmul(xv.x, xv.y, diag_inv<T>(da, e, s), yr, yi);
     gt_apply<T, SP>(G, help, e, s, yr, yi);
     T2 o;
     o.x = yr;
@@ -1686,9 +1701,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
   auto elem = [&](long long t, T2 zv, T2 av) {
     const int e = int(t / SP);
     const T2 an = caxpy(zv, av, br, bi);
-    const T2 d = diag_inv<T>(da, e, s);
-    T yr = an.x * d.x - an.y * d.y;
-    T yi = an.x * d.y + an.y * d.x;
+    T yr, yi;
+    pc_cmul(an.x, an.y, diag_inv<T>(da, e, s), yr, yi);
     gt_apply<T, SP>(G, help, e, s, yr, yi);
     T2 o;
     o.x = yr;
@@ -1714,6 +1728,206 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
     }
     if (ok0) elem(t0, z0, a0);
     if (ok1) elem(t1, z1, a1);
+  }
+  block_partials<SP, 2>(v, partials);
+}
+
+// Fused preconditioner pass on a box mesh held in the grid order (16 systems): the node pass
+// (help = D_N^-1 G a) and the edge pass (a = z + beta a, w = D_A^-1 a + G^T help, partial (a, w))
+// of one iteration in one kernel, with help kept in LDS instead of written to and re-read from
+// HBM, and each edge's z and a read once instead of three times. A tile reads the a of edges
+// that neighbouring tiles own, so the new a goes to the other of two buffers (`ain` holds the
+// previous iteration's, `aout` receives this one's; they alternate with the iterations).
+//
+// Every node (i, j, k) owns its forward edges: the x-edge to (i+1, j, k), the y-edge to
+// (i, j+1, k) and the z-edge to (i, j, k+1), where they exist. A workgroup takes a tile of
+// kPgT x kPgT nodes in (i, j) and sweeps a chunk of node levels in k. At level L it forms a
+// for the edges it needs, help(L) of the tile's nodes and of the halo nodes one step beyond in
+// +i and +j (computed again by the neighbouring tile) into an LDS plane, then w of the tile's
+// x- and y-edges at level L and of its z-edges at level L - 1 (whose upper node's help is
+// help(L)). The z-edge below a node is the z-edge above it one level down: its a is kept in a
+// register from the previous level. G's rows of a box mesh are the six edges of an interior node
+// in ascending reference edge order (z below, y below, x below, x above, y above, z above, the
+// first three +1/2, the others -1/2), G^T's rows the edge's lower then upper node (-1/2, +1/2);
+// the host checks the uploaded G against this before the kernel is used, so the sums run in
+// the node and edge kernels' order and give their bits.
+constexpr int kPgT = 8;                        // tile side in nodes
+constexpr int kPgE = kPgT + 1;                 // side of the tile with its halo
+constexpr int kPgSlots = kBlock / 16;          // node slots of a workgroup (16 systems each)
+constexpr int kPgOwn = kPgT * kPgT / kPgSlots; // owned nodes per slot and level
+constexpr int kPgHalo = (2 * kPgT + 1 + kPgSlots - 1) / kPgSlots;  // halo nodes per slot
+
+struct PgView {
+  int nx, ny, nz;      // nodes per axis of the box
+  int nti, ntj;        // tiles in i and j
+  int kc, nkc;         // node levels per chunk, chunks
+  long long ntasks;
+  int nb;
+};
+
+template <class T, int SP>
+__global__ __launch_bounds__(kBlock) void k_pc_grid(
+    PgView P, const typename Vec2<T>::type* __restrict__ z,
+    const typename Vec2<T>::type* __restrict__ ain, typename Vec2<T>::type* __restrict__ aout,
+    typename Vec2<T>::type* __restrict__ w, const typename Vec2<T>::type* __restrict__ dn,
+    DiagA da, const SysScalars* __restrict__ sc, const int* __restrict__ nrun,
+    double* partials) {
+  static_assert(SP == 16, "16 systems: four node slots per wave");
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 hp[2][kPgE * kPgE * SP];
+  if (*nrun == 0) return;
+  const int s = int(threadIdx.x) % SP, q = int(threadIdx.x) / SP;
+  const bool run = sc[s].status == ST_RUNNING;
+  const double br = run ? sc[s].be_re : 0.0, bi = run ? sc[s].be_im : 0.0;
+  const long long vb = (long long)P.nb * SP * (long long)sizeof(T2);
+  const __amdgpu_buffer_rsrc_t zr = buf_rsrc(z, vb), ar = buf_rsrc(ain, vb);
+  const __amdgpu_buffer_rsrc_t aw = buf_rsrc(aout, vb), wr = buf_rsrc(w, vb);
+  const long long m = (long long)(P.nx - 2) * (P.ny - 2) * (P.nz - 2);
+  const __amdgpu_buffer_rsrc_t dr = buf_rsrc(dn, m * SP * (long long)sizeof(T2));
+  const GridDims G{P.nx, P.ny, P.nz};
+  auto store = [&](__amdgpu_buffer_rsrc_t r, unsigned off, T2 val) {
+    if constexpr (sizeof(T2) == 16)
+      __builtin_amdgcn_raw_buffer_store_b128(
+          __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, val), r, int(off), 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, val), r, int(off), 0, 0);
+  };
+  // byte offset of system s of edge (d, i, j, k), or kOob where the edge does not exist
+  auto eoff = [&](int d, int i, int j, int k) -> unsigned {
+    const bool ok = i >= 0 && j >= 0 && k >= 0 && i < G.nxd(d) && j < G.nyd(d) && k < G.nzd(d);
+    const long long row = G.base(d) + ((long long)k * G.nyd(d) + j) * G.nxd(d) + i;
+    return ok ? unsigned((row * SP + s) * (long long)sizeof(T2)) : kOob;
+  };
+  auto interior = [&](int i, int j, int k) {
+    return i >= 1 && j >= 1 && k >= 1 && i <= P.nx - 2 && j <= P.ny - 2 && k <= P.nz - 2;
+  };
+  // a = z + beta a of an edge (0 where the edge does not exist: both loads return 0)
+  auto anew = [&](unsigned off) {
+    return caxpy(buffer_gather<T2>(zr, off), buffer_gather<T2>(ar, off), br, bi);
+  };
+  auto dinv = [&](int d, int i, int j, int k) {
+    T2 o;
+    o.x = o.y = T(0);
+    if (i < G.nxd(d) && j < G.nyd(d) && k < G.nzd(d)) {
+      const long long row = G.base(d) + ((long long)k * G.nyd(d) + j) * G.nxd(d) + i;
+      o = diag_inv<T>(da, int(row), s);
+    }
+    return o;
+  };
+  const T half = T(0.5);
+  double v[2] = {0.0, 0.0};
+  TileWalk tw(P.ntasks);
+  for (long long task = tw.first; task < tw.last; task += tw.step) {
+    const int ti = int(task % P.nti);
+    const long long r1 = task / P.nti;
+    const int tj = int(r1 % P.ntj);
+    const int kcI = int(r1 / P.ntj);
+    const int i0 = ti * kPgT, j0 = tj * kPgT;
+    const int k0 = kcI * P.kc, k1 = min(k0 + P.kc, P.nz);  // owned node levels [k0, k1)
+    const int Lend = min(k1, P.nz - 1);                    // help levels k0 .. Lend
+    // the slot's nodes: owned o = q + kPgSlots*r of the tile, then halo nodes (the column
+    // i = i0 + kPgT, then the row j = j0 + kPgT)
+    int oi[kPgOwn], oj[kPgOwn], hi_[kPgHalo], hj_[kPgHalo];
+    bool hv[kPgHalo];
+#pragma unroll
+    for (int r = 0; r < kPgOwn; ++r) {
+      const int o = q + kPgSlots * r;
+      oi[r] = i0 + o % kPgT;
+      oj[r] = j0 + o / kPgT;
+    }
+#pragma unroll
+    for (int r = 0; r < kPgHalo; ++r) {
+      const int hh = q + kPgSlots * r;
+      hv[r] = hh < 2 * kPgT + 1;
+      hi_[r] = hh <= kPgT ? i0 + kPgT : i0 + (hh - kPgT - 1);
+      hj_[r] = hh <= kPgT ? j0 + hh : j0 + kPgT;
+    }
+    // a of the z-edge below each node at the first level (kept from level to level after)
+    T2 zo[kPgOwn], zh[kPgHalo];
+#pragma unroll
+    for (int r = 0; r < kPgOwn; ++r) zo[r] = anew(eoff(2, oi[r], oj[r], k0 - 1));
+#pragma unroll
+    for (int r = 0; r < kPgHalo; ++r) zh[r] = anew(eoff(2, hi_[r], hj_[r], k0 - 1));
+    for (int L = k0; L <= Lend; ++L) {
+      T2* cur = hp[L & 1];
+      const T2* prv = hp[(L + 1) & 1];
+      // help of a node at level L from the a of its six edges (zb: the z-edge below)
+      auto help_of = [&](int i, int j, T2 zb, T2 xp, T2 yp, T2 zp) {
+        T2 o;
+        o.x = o.y = T(0);
+        if (interior(i, j, L)) {
+          const T2 xm = anew(eoff(0, i - 1, j, L)), ym = anew(eoff(1, i, j - 1, L));
+          T hr = T(0), hi = T(0);
+          pc_acc(hr, hi, half, zb);
+          pc_acc(hr, hi, half, ym);
+          pc_acc(hr, hi, half, xm);
+          pc_acc(hr, hi, -half, xp);
+          pc_acc(hr, hi, -half, yp);
+          pc_acc(hr, hi, -half, zp);
+          const long long mi =
+              (i - 1) + (long long)(P.nx - 2) * ((j - 1) + (long long)(P.ny - 2) * (L - 1));
+          pc_cmul(hr, hi, buffer_gather<T2>(dr, unsigned((mi * SP + s) * (long long)sizeof(T2))),
+                  o.x, o.y);
+        }
+        return o;
+      };
+      T2 xo[kPgOwn], yo[kPgOwn], zn[kPgOwn];
+#pragma unroll
+      for (int r = 0; r < kPgOwn; ++r) {
+        xo[r] = anew(eoff(0, oi[r], oj[r], L));
+        yo[r] = anew(eoff(1, oi[r], oj[r], L));
+        zn[r] = anew(eoff(2, oi[r], oj[r], L));
+        cur[((oj[r] - j0) * kPgE + (oi[r] - i0)) * SP + s] = help_of(oi[r], oj[r], zo[r], xo[r],
+                                                                      yo[r], zn[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < kPgHalo; ++r) {
+        if (hv[r]) {
+          const T2 xh = anew(eoff(0, hi_[r], hj_[r], L)), yh = anew(eoff(1, hi_[r], hj_[r], L));
+          const T2 zhn = anew(eoff(2, hi_[r], hj_[r], L));
+          cur[((hj_[r] - j0) * kPgE + (hi_[r] - i0)) * SP + s] =
+              help_of(hi_[r], hj_[r], zh[r], xh, yh, zhn);
+          zh[r] = zhn;
+        }
+      }
+      __syncthreads();  // help(L) of the tile and its halo is in `cur`
+      // w = D_A^-1 a + G^T help of the owned edges: x and y at level L (if L is owned), z at
+      // level L - 1 (its upper node is at L)
+#pragma unroll
+      for (int r = 0; r < kPgOwn; ++r) {
+        const int i = oi[r], j = oj[r];
+        const int c = ((j - j0) * kPgE + (i - i0)) * SP + s;
+        const bool own_n = interior(i, j, L);
+        const T2 hn = cur[c];
+        auto finish = [&](unsigned off, T2 an, T2 d, bool lo_in, T2 hlo, bool up_in, T2 hup) {
+          T yr, yi;
+          pc_cmul(an.x, an.y, d, yr, yi);
+          if (lo_in) pc_acc(yr, yi, -half, hlo);
+          if (up_in) pc_acc(yr, yi, half, hup);
+          T2 o;
+          o.x = yr;
+          o.y = yi;
+          store(aw, off, an);
+          store(wr, off, o);
+          if (off != kOob) {
+            v[0] += double(an.x) * double(yr) - double(an.y) * double(yi);
+            v[1] += double(an.x) * double(yi) + double(an.y) * double(yr);
+          }
+        };
+        if (L < k1) {
+          finish(eoff(0, i, j, L), xo[r], dinv(0, i, j, L), own_n, hn, interior(i + 1, j, L),
+                 cur[c + SP]);
+          finish(eoff(1, i, j, L), yo[r], dinv(1, i, j, L), own_n, hn, interior(i, j + 1, L),
+                 cur[c + kPgE * SP]);
+        }
+        if (L > k0)
+          finish(eoff(2, i, j, L - 1), zo[r], dinv(2, i, j, L - 1), interior(i, j, L - 1), prv[c],
+                 own_n, hn);
+        zo[r] = zn[r];
+      }
+      __syncthreads();  // every read of `prv` is done before the next level overwrites it
+    }
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -1786,7 +2000,12 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
           T2 yv = y0;
           yv.x += wm * (E.x.x - yv.x);
           yv.y += wm * (E.x.y - yv.y);
-          ym[t] = run ? yv : y0;  // the smoothed iterate of a lane not running stays as it is
+          // the smoothed iterate of a lane not running stays as it is (a select per component:
+          // a select of the struct went through scratch memory)
+          T2 yo;
+          yo.x = run ? yv.x : y0.x;
+          yo.y = run ? yv.y : y0.y;
+          ym[t] = yo;
         }
         const T ur = E.r.x - smv.x, ui = E.r.y - smv.y;
         smv.x += wm * ur;
@@ -2270,6 +2489,10 @@ struct mtcocr_handle {
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0, g_nb = 0;
   DevBuf gcol, gtcol, gval64, gtval64, gval32, gtval32;
+  // G is the whole box's gradient in the grid order (gradient_on_grid): 16-system batches then
+  // run the fused preconditioner kernel k_pc_grid, with its node levels per chunk per precision
+  bool pg_ok = false;
+  int pg_kc[2] = {8, 8};
 
   // batch state
   int nsys = 0, sp = 0;
@@ -2277,6 +2500,7 @@ struct mtcocr_handle {
   DevBuf b64, x64, r64;  // FP64 rhs / solution / outer residual
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
+  DevBuf va2;  // the second buffer of a, alternating with va, for k_pc_grid
   // the same vectors for a narrowed batch (swapped with the above by compaction)
   DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
   DevBuf cmap;  // slot maps of compaction / scatter
@@ -2285,7 +2509,7 @@ struct mtcocr_handle {
   DevBuf sg_ce, sg_xyz, sg_coef, sg_u, sg_v, sg_out;
   int work_prec = -1, work_sp = 0;
   hipGraphExec_t graph = nullptr;
-  int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1;
+  int graph_prec = -1, graph_sp = 0, graph_chunk = 0, graph_maxiter = -1, graph_pg = -1;
   // launch shapes (workgroup caps of the grid-stride kernels) per vector precision (FP64,
   // FP32) and batch width SP = 1..64 (index log2 SP), set by mtcocr_set_launch / autotune
   int cap_spmv[2][7], cap_vec[2][7], cap_node[2][7];
@@ -2573,6 +2797,38 @@ GView<T> g_view(mtcocr_handle* h) {
   return g;
 }
 
+// The fused grid preconditioner applies to 16-system batches on a box mesh in the grid order
+// whose G has the box's structure, with both a buffers allocated, byte offsets of the batch
+// vectors within 32 bits, and unless MTCOCR_PC_SPLIT=1 asks for the node and edge kernels.
+inline bool pg_applies(const mtcocr_handle* h, int sp, size_t es) {
+  if (sp != 16 || !h->pg_ok || !h->have_g) return false;
+  if ((long long)h->nb * sp * (long long)es >= (long long)kOob) return false;
+  if (h->va2.bytes < (size_t)h->nb * sp * es || h->va.bytes < (size_t)h->nb * sp * es) return false;
+  const char* e = std::getenv("MTCOCR_PC_SPLIT");
+  return !(e && e[0] == '1');
+}
+
+inline PgView pg_view(const mtcocr_handle* h, int kc) {
+  PgView v;
+  v.nx = h->grid.nx;
+  v.ny = h->grid.ny;
+  v.nz = h->grid.nz;
+  v.nti = (v.nx + kPgT - 1) / kPgT;
+  v.ntj = (v.ny + kPgT - 1) / kPgT;
+  v.kc = std::max(1, std::min(kc, v.nz));
+  v.nkc = (v.nz + v.kc - 1) / v.kc;
+  v.ntasks = (long long)v.nkc * v.ntj * v.nti;
+  v.nb = h->nb;
+  return v;
+}
+
+// one workgroup per task up to the partials' capacity, a multiple of the XCD count
+inline int pg_grid(const PgView& v) {
+  long long g = std::min<long long>(v.ntasks, kMaxBlocks);
+  g = std::max<long long>(g, kXcd);
+  return round_up(int(g), kXcd);
+}
+
 // COCR engine over SP systems in storage precision T.
 template <class T, int SP>
 struct Engine {
@@ -2580,13 +2836,17 @@ struct Engine {
   mtcocr_handle* h;
   EllView<double> A;
   GView<T> G;
-  T2 *x, *x2, *b, *r, *p, *s, *z, *a, *w, *ym, *sm, *help, *dn;
+  T2 *x, *x2, *b, *r, *p, *s, *z, *a, *a2, *w, *ym, *sm, *help, *dn;
   DiagA da;
   SysScalars* sc;
   int* nrun;
   double *part, *part2;
   const double* omega;
   int nb, m, grid_s, grid_e, grid_n, grid_u, layout;
+  // the fused grid preconditioner (k_pc_grid): used, its view and grid
+  bool use_pg = false;
+  PgView pg;
+  int grid_pg = 0;
 
   explicit Engine(mtcocr_handle* hh) : h(hh) {
     A = ell_view<double>(h);
@@ -2620,6 +2880,12 @@ struct Engine {
     grid_u = grid_for((long long)nb * SP, h->cap_upd[q][k]);
     grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
     layout = h->spmv_layout[q][k];
+    a2 = h->va2.as<T2>();
+    use_pg = pg_applies(h, SP, sizeof(T2));
+    if (use_pg) {
+      pg = pg_view(h, h->pg_kc[q]);
+      grid_pg = pg_grid(pg);
+    }
   }
 
   // The MRS-smoothed iterate: the FP32 inner solve of the mixed solver returns it as its
@@ -2675,12 +2941,25 @@ struct Engine {
   // one COCR iteration (stages 5-8); `odd` = the iteration's index is odd (it reads x from
   // buffer 0 and writes buffer 1), which alternates within a captured chunk of iterations
   void iteration(hipStream_t st, int maxiter, bool odd) {
-    k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
-    k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, w, da, help, sc, nrun, part);
     int nr;
-    const double* pp = partials<SP * 2>(st, grid_e, nr);
+    const double* pp;
+    T2* acur = a;  // a of this iteration after the preconditioner pass
+    if constexpr (SP == 16) {
+      if (use_pg) {
+        // a alternates between the two buffers: odd iterations read a and write a2
+        acur = odd ? a2 : a;
+        k_pc_grid<T, SP><<<grid_pg, kBlock, 0, st>>>(pg, z, odd ? a : a2, acur, w, dn, da, sc,
+                                                     nrun, part);
+        pp = partials<SP * 2>(st, grid_pg, nr);
+      }
+    }
+    if (!use_pg) {
+      k_pc_node<T, SP, true><<<grid_n, kBlock, 0, st>>>(G, z, a, dn, help, sc, nrun);
+      k_pc_edge_iter<T, SP><<<grid_e, kBlock, 0, st>>>(G, z, a, w, da, help, sc, nrun, part);
+      pp = partials<SP * 2>(st, grid_e, nr);
+    }
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, a, s, w,
+    k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, acur, s, w,
                                                  ym_or_null(), sm, sc, nrun, part, nb);
     pp = partials<SP * 3>(st, grid_u, nr);
     s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
@@ -2706,6 +2985,7 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
                     &h->vx2, &h->vb};
   for (DevBuf* b : vecs)
     if ((rc = ensure(h, *b, nv))) return rc;
+  if (sp >= 16 && h->pg_ok && (rc = ensure(h, h->va2, (size_t)h->nb * 16 * es))) return rc;
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
   if ((rc = ensure(h, h->vdn, nm))) return rc;
   h->work_prec = prec;
@@ -2726,7 +3006,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_bel
   const int chunk = 8;
   if (use_graphs()) {
     if (!(h->graph && h->graph_prec == prec && h->graph_sp == SP && h->graph_chunk == chunk &&
-          h->graph_maxiter == maxiter)) {
+          h->graph_maxiter == maxiter && h->graph_pg == int(E.use_pg))) {
       if (h->graph) {
         (void)hipGraphExecDestroy(h->graph);
         h->graph = nullptr;
@@ -2742,6 +3022,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_bel
       h->graph_sp = SP;
       h->graph_chunk = chunk;
       h->graph_maxiter = maxiter;
+      h->graph_pg = int(E.use_pg);
     }
   }
   // every running system leaves RUNNING after at most maxiter + 1 iterations; the bound below
@@ -3359,6 +3640,8 @@ int mtcocr_set_gradient_mesh(mtcocr_handle* h, int kpar, int n_edges, int n_node
   h->wg = G.wg;
   h->wgt = G.wgt;
   h->g_nb = n_edges;
+  h->pg_ok = h->grid_order && gradient_on_grid(G, h->grid);
+  drop_graph(h);  // captured iterations hold the preconditioner path
   h->have_g = true;
   h->work_prec = -1;
   return 0;

@@ -182,6 +182,78 @@ def test_stencil_batched_solve(O, M, c1_model, c1_mesh):
         assert rel(xl[k], xs[k]) <= 1e-5
 
 
+@pytest.mark.parametrize("shape", [None, (13, 10, 9)])
+def test_fused_grid_preconditioner(O, M, c1_model, shape, monkeypatch):
+    """A 16-system batch on a box mesh in the grid order runs the preconditioner as one fused
+    kernel (node and edge passes together, help in LDS, a alternating between two buffers).
+    Against the node and edge kernels (MTCOCR_PC_SPLIT=1) on the same batch, whose arithmetic
+    per element it shares (only the dot products' partial sums group differently): x after
+    three iterations within ten times the split path's own rounding spread (its two stencil
+    SpMV kernels); solved, FP64 iterations within 2% + 2 and x within 1e-5, mixed true
+    residuals below 1e-8; and the oracle on two systems. Config 1's box (25^3 nodes) and a
+    14 x 11 x 10-node box: neither is a multiple of the 8 x 8 node tile."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    m = c1_model if shape is None else M.random_model(*shape, seed=7, h=100.0, n_pad=2, n_air=3)
+    ma = M.mesh_arrays(m)
+    say(f"fused grid preconditioner on {m.shape_cells} cells, 16 systems")
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(m)
+    prs = np.stack([M.assemble_rhs(m, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    runs = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("MTCOCR_PC_SPLIT", split)
+        with Solver(0) as B:
+            B.set_mesh_order(ma, ORDER_GRID)
+            B.set_matrix_ks(p, kd, kg, sd, sg)
+            B.set_gradient(ma)
+            for prec in ("fp64", "mixed"):
+                runs[split, prec] = B.solve_batch(om, prs, eps=1e-8, precision=prec)
+            runs[split, "3 iterations"] = B.solve_batch(om, prs, eps=1e-8, maxiter=3)
+            if split == "1":
+                # the same path with the other stencil SpMV kernel: the partial sums of (z, s)
+                # group differently, which sets the rounding spread of x after 3 iterations
+                B.set_launch(spmv_layout="stencil")
+                runs["1b", "3 iterations"] = B.solve_batch(om, prs, eps=1e-8, maxiter=3)
+    # after three iterations x depends on w through s = s - alpha w: a wrong w would move it by
+    # O(1); the fused path stays within the rounding spread of the split path's own kernels
+    # (the only difference is the grouping of the (a, w) partial sums)
+    (x3f, s3f), (x3s, s3s) = runs["0", "3 iterations"], runs["1", "3 iterations"]
+    x3b, s3b = runs["1b", "3 iterations"]
+    d3 = max(rel(x3f[k], x3s[k]) for k in range(len(cases)))
+    dref = max(rel(x3b[k], x3s[k]) for k in range(len(cases)))
+    say(f"  after 3 iterations: x max rel diff {d3:.2e} (split path, two SpMV kernels: "
+        f"{dref:.2e})")
+    assert [q["iter"] for q in s3f] == [q["iter"] for q in s3s] == [q["iter"] for q in s3b]
+    assert d3 <= max(10 * dref, 1e-12)
+    for prec in ("fp64", "mixed"):
+        (xf, sf), (xs, ss) = runs["0", prec], runs["1", prec]
+        d = [rel(xf[k], xs[k]) for k in range(len(cases))]
+        say(f"  {prec}: fused iterations {[q['iter'] for q in sf]}, split "
+            f"{[q['iter'] for q in ss]}, x max rel diff {max(d):.2e}")
+        for k in range(len(cases)):
+            assert sf[k]["req"] == 0 and ss[k]["req"] == 0
+            if prec == "fp64":
+                assert abs(sf[k]["iter"] - ss[k]["iter"]) <= 2 + 0.02 * ss[k]["iter"]
+                assert d[k] <= 1e-5, (prec, k, d[k])
+            else:
+                # the mixed solver's passes and finish follow its stall tests, which rounding
+                # moves, and its answer is held by its FP64 true residual
+                assert sf[k]["true_rel_res"] < 1e-8 and ss[k]["true_rel_res"] < 1e-8
+    # against the oracle: the small box is far less well conditioned (its rounding spread after
+    # three iterations is 1e5 times config 1's), so its x is held looser
+    xf, sf = runs["0", "fp64"]
+    for k in (0, 15):
+        f, a = cases[k]
+        xo, so, _ = O.pcocr(M.assemble(m, f, a), ma, eps=1e-8)
+        assert abs(sf[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
+        assert rel(xf[k], xo) <= (1e-5 if shape is None else 1e-4)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
 def test_preconditioner_parity(S, O, M, c1_system, c1_mesh, prec, tol):
     say(f"preconditioner {prec}")
