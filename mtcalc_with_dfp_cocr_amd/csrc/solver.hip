@@ -1741,21 +1741,28 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
 //
 // Every node (i, j, k) owns its forward edges: the x-edge to (i+1, j, k), the y-edge to
 // (i, j+1, k) and the z-edge to (i, j, k+1), where they exist. A workgroup takes a tile of
-// kPgT x kPgT nodes in (i, j) and sweeps a chunk of node levels in k. At level L it forms a
+// kPgTI x kPgTJ nodes in (i, j) and sweeps a chunk of node levels in k. At level L it forms a
 // for the edges it needs, help(L) of the tile's nodes and of the halo nodes one step beyond in
 // +i and +j (computed again by the neighbouring tile) into an LDS plane, then w of the tile's
 // x- and y-edges at level L and of its z-edges at level L - 1 (whose upper node's help is
-// help(L)). The z-edge below a node is the z-edge above it one level down: its a is kept in a
+// help(L)). Three planes rotate, so one barrier per level suffices: the plane a level writes
+// was last read two levels before, behind the previous level's barrier. An 8 x 4 tile (two
+// owned nodes per slot, 164 VGPRs, three workgroups per CU) ran 2% faster per solve than 8 x 8
+// (four nodes per slot, two workgroups per CU), and chunks of 8 levels faster than 4, 6, 12
+// or 16. The z-edge below a node is the z-edge above it one level down: its a is kept in a
 // register from the previous level. G's rows of a box mesh are the six edges of an interior node
 // in ascending reference edge order (z below, y below, x below, x above, y above, z above, the
 // first three +1/2, the others -1/2), G^T's rows the edge's lower then upper node (-1/2, +1/2);
 // the host checks the uploaded G against this before the kernel is used, so the sums run in
 // the node and edge kernels' order and give their bits.
-constexpr int kPgT = 8;                        // tile side in nodes
-constexpr int kPgE = kPgT + 1;                 // side of the tile with its halo
+constexpr int kPgTI = 8, kPgTJ = 4;            // tile sides in nodes (i, j)
+constexpr int kPgE = kPgTI + 1;                // row length of the tile with its halo
+constexpr int kPgEJ = kPgTJ + 1;               // rows of the tile with its halo
 constexpr int kPgSlots = kBlock / 16;          // node slots of a workgroup (16 systems each)
-constexpr int kPgOwn = kPgT * kPgT / kPgSlots; // owned nodes per slot and level
-constexpr int kPgHalo = (2 * kPgT + 1 + kPgSlots - 1) / kPgSlots;  // halo nodes per slot
+constexpr int kPgOwn = kPgTI * kPgTJ / kPgSlots;  // owned nodes per slot and level
+constexpr int kPgNH = kPgTJ + 1 + kPgTI;          // halo nodes: the column i0 + TI, the row j0 + TJ
+constexpr int kPgHalo = (kPgNH + kPgSlots - 1) / kPgSlots;  // halo nodes per slot
+static_assert(kPgTI * kPgTJ % kPgSlots == 0, "owned nodes spread evenly over the slots");
 
 struct PgView {
   int nx, ny, nz;      // nodes per axis of the box
@@ -1774,7 +1781,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
     double* partials) {
   static_assert(SP == 16, "16 systems: four node slots per wave");
   using T2 = typename Vec2<T>::type;
-  __shared__ T2 hp[2][kPgE * kPgE * SP];
+  __shared__ T2 hp[3][kPgE * kPgEJ * SP];
   if (*nrun == 0) return;
   const int s = int(threadIdx.x) % SP, q = int(threadIdx.x) / SP;
   const bool run = sc[s].status == ST_RUNNING;
@@ -1823,25 +1830,25 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
     const long long r1 = task / P.nti;
     const int tj = int(r1 % P.ntj);
     const int kcI = int(r1 / P.ntj);
-    const int i0 = ti * kPgT, j0 = tj * kPgT;
+    const int i0 = ti * kPgTI, j0 = tj * kPgTJ;
     const int k0 = kcI * P.kc, k1 = min(k0 + P.kc, P.nz);  // owned node levels [k0, k1)
     const int Lend = min(k1, P.nz - 1);                    // help levels k0 .. Lend
     // the slot's nodes: owned o = q + kPgSlots*r of the tile, then halo nodes (the column
-    // i = i0 + kPgT, then the row j = j0 + kPgT)
+    // i = i0 + kPgTI, then the row j = j0 + kPgTJ)
     int oi[kPgOwn], oj[kPgOwn], hi_[kPgHalo], hj_[kPgHalo];
     bool hv[kPgHalo];
 #pragma unroll
     for (int r = 0; r < kPgOwn; ++r) {
       const int o = q + kPgSlots * r;
-      oi[r] = i0 + o % kPgT;
-      oj[r] = j0 + o / kPgT;
+      oi[r] = i0 + o % kPgTI;
+      oj[r] = j0 + o / kPgTI;
     }
 #pragma unroll
     for (int r = 0; r < kPgHalo; ++r) {
       const int hh = q + kPgSlots * r;
-      hv[r] = hh < 2 * kPgT + 1;
-      hi_[r] = hh <= kPgT ? i0 + kPgT : i0 + (hh - kPgT - 1);
-      hj_[r] = hh <= kPgT ? j0 + hh : j0 + kPgT;
+      hv[r] = hh < kPgNH;
+      hi_[r] = hh <= kPgTJ ? i0 + kPgTI : i0 + (hh - kPgTJ - 1);
+      hj_[r] = hh <= kPgTJ ? j0 + hh : j0 + kPgTJ;
     }
     // a of the z-edge below each node at the first level (kept from level to level after)
     T2 zo[kPgOwn], zh[kPgHalo];
@@ -1850,8 +1857,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
 #pragma unroll
     for (int r = 0; r < kPgHalo; ++r) zh[r] = anew(eoff(2, hi_[r], hj_[r], k0 - 1));
     for (int L = k0; L <= Lend; ++L) {
-      T2* cur = hp[L & 1];
-      const T2* prv = hp[(L + 1) & 1];
+      T2* cur = hp[L % 3];
+      const T2* prv = hp[(L + 2) % 3];
       // help of a node at level L from the a of its six edges (zb: the z-edge below)
       auto help_of = [&](int i, int j, T2 zb, T2 xp, T2 yp, T2 zp) {
         T2 o;
@@ -1926,8 +1933,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
                  own_n, hn);
         zo[r] = zn[r];
       }
-      __syncthreads();  // every read of `prv` is done before the next level overwrites it
     }
+    __syncthreads();  // the task's last planes are read before the next task writes its first
   }
   block_partials<SP, 2>(v, partials);
 }
@@ -2813,8 +2820,8 @@ inline PgView pg_view(const mtcocr_handle* h, int kc) {
   v.nx = h->grid.nx;
   v.ny = h->grid.ny;
   v.nz = h->grid.nz;
-  v.nti = (v.nx + kPgT - 1) / kPgT;
-  v.ntj = (v.ny + kPgT - 1) / kPgT;
+  v.nti = (v.nx + kPgTI - 1) / kPgTI;
+  v.ntj = (v.ny + kPgTJ - 1) / kPgTJ;
   v.kc = std::max(1, std::min(kc, v.nz));
   v.nkc = (v.nz + v.kc - 1) / v.kc;
   v.ntasks = (long long)v.nkc * v.ntj * v.nti;
