@@ -1732,7 +1732,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
   block_partials<SP, 2>(v, partials);
 }
 
-// Fused preconditioner pass on a box mesh held in the grid order (16 systems): the node pass
+// Fused preconditioner pass on a box mesh held in the grid order (8 or 16 systems): the node pass
 // (help = D_N^-1 G a) and the edge pass (a = z + beta a, w = D_A^-1 a + G^T help, partial (a, w))
 // of one iteration in one kernel, with help kept in LDS instead of written to and re-read from
 // HBM, and each edge's z and a read once instead of three times. A tile reads the a of edges
@@ -1758,11 +1758,16 @@ __global__ __launch_bounds__(kBlock) void k_pc_edge_iter(
 constexpr int kPgTI = 8, kPgTJ = 4;            // tile sides in nodes (i, j)
 constexpr int kPgE = kPgTI + 1;                // row length of the tile with its halo
 constexpr int kPgEJ = kPgTJ + 1;               // rows of the tile with its halo
-constexpr int kPgSlots = kBlock / 16;          // node slots of a workgroup (16 systems each)
-constexpr int kPgOwn = kPgTI * kPgTJ / kPgSlots;  // owned nodes per slot and level
 constexpr int kPgNH = kPgTJ + 1 + kPgTI;          // halo nodes: the column i0 + TI, the row j0 + TJ
-constexpr int kPgHalo = (kPgNH + kPgSlots - 1) / kPgSlots;  // halo nodes per slot
-static_assert(kPgTI * kPgTJ % kPgSlots == 0, "owned nodes spread evenly over the slots");
+// per batch width SP: node slots of a workgroup (SP lanes each), owned and halo nodes per slot
+template <int SP>
+constexpr int pg_slots() { return kBlock / SP; }
+template <int SP>
+constexpr int pg_own() { return kPgTI * kPgTJ / pg_slots<SP>(); }
+template <int SP>
+constexpr int pg_halo() { return (kPgNH + pg_slots<SP>() - 1) / pg_slots<SP>(); }
+// the batch widths the fused kernel serves (a whole number of owned nodes per slot)
+constexpr bool pg_width(int sp) { return sp == 8 || sp == 16; }
 
 struct PgView {
   int nx, ny, nz;      // nodes per axis of the box
@@ -1779,7 +1784,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
     typename Vec2<T>::type* __restrict__ w, const typename Vec2<T>::type* __restrict__ dn,
     DiagA da, const SysScalars* __restrict__ sc, const int* __restrict__ nrun,
     double* partials) {
-  static_assert(SP == 16, "16 systems: four node slots per wave");
+  static_assert(pg_width(SP) && kPgTI * kPgTJ % pg_slots<SP>() == 0,
+                "a whole number of owned nodes per slot");
+  constexpr int kPgSlots = pg_slots<SP>(), kPgOwn = pg_own<SP>(), kPgHalo = pg_halo<SP>();
   using T2 = typename Vec2<T>::type;
   __shared__ T2 hp[3][kPgE * kPgEJ * SP];
   if (*nrun == 0) return;
@@ -2496,7 +2503,7 @@ struct mtcocr_handle {
   bool have_g = false;
   int m = 0, wg = 0, wgt = 0, g_nb = 0;
   DevBuf gcol, gtcol, gval64, gtval64, gval32, gtval32;
-  // G is the whole box's gradient in the grid order (gradient_on_grid): 16-system batches then
+  // G is the whole box's gradient in the grid order (gradient_on_grid): batches of 8 or 16 then
   // run the fused preconditioner kernel k_pc_grid, with its node levels per chunk per precision
   bool pg_ok = false;
   int pg_kc[2] = {8, 8};
@@ -2804,11 +2811,11 @@ GView<T> g_view(mtcocr_handle* h) {
   return g;
 }
 
-// The fused grid preconditioner applies to 16-system batches on a box mesh in the grid order
+// The fused grid preconditioner applies to batches of 8 or 16 systems on a box mesh in the grid order
 // whose G has the box's structure, with both a buffers allocated, byte offsets of the batch
 // vectors within 32 bits, and unless MTCOCR_PC_SPLIT=1 asks for the node and edge kernels.
 inline bool pg_applies(const mtcocr_handle* h, int sp, size_t es) {
-  if (sp != 16 || !h->pg_ok || !h->have_g) return false;
+  if (!pg_width(sp) || !h->pg_ok || !h->have_g) return false;
   if ((long long)h->nb * sp * (long long)es >= (long long)kOob) return false;
   if (h->va2.bytes < (size_t)h->nb * sp * es || h->va.bytes < (size_t)h->nb * sp * es) return false;
   const char* e = std::getenv("MTCOCR_PC_SPLIT");
@@ -2951,7 +2958,7 @@ struct Engine {
     int nr;
     const double* pp;
     T2* acur = a;  // a of this iteration after the preconditioner pass
-    if constexpr (SP == 16) {
+    if constexpr (pg_width(SP)) {
       if (use_pg) {
         // a alternates between the two buffers: odd iterations read a and write a2
         acur = odd ? a2 : a;
@@ -2992,7 +2999,10 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
                     &h->vx2, &h->vb};
   for (DevBuf* b : vecs)
     if ((rc = ensure(h, *b, nv))) return rc;
-  if (sp >= 16 && h->pg_ok && (rc = ensure(h, h->va2, (size_t)h->nb * 16 * es))) return rc;
+  // the second a buffer of the fused grid preconditioner (batches of 8 or 16 systems; wider
+  // FP64 batches run in groups of 16)
+  if (sp >= 8 && h->pg_ok && (rc = ensure(h, h->va2, (size_t)h->nb * std::min(sp, 16) * es)))
+    return rc;
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
   if ((rc = ensure(h, h->vdn, nm))) return rc;
   h->work_prec = prec;

@@ -182,24 +182,24 @@ def test_stencil_batched_solve(O, M, c1_model, c1_mesh):
         assert rel(xl[k], xs[k]) <= 1e-5
 
 
-@pytest.mark.parametrize("shape", [None, (13, 10, 9)])
-def test_fused_grid_preconditioner(O, M, c1_model, shape, monkeypatch):
-    """A 16-system batch on a box mesh in the grid order runs the preconditioner as one fused
-    kernel (node and edge passes together, help in LDS, a alternating between two buffers).
-    Against the node and edge kernels (MTCOCR_PC_SPLIT=1) on the same batch, whose arithmetic
-    per element it shares (only the dot products' partial sums group differently): x after
-    three iterations within ten times the split path's own rounding spread (its two stencil
-    SpMV kernels); solved, FP64 iterations within 2% + 2 and x within 1e-5, mixed true
-    residuals below 1e-8; and the oracle on two systems. Config 1's box (25^3 nodes) and a
-    14 x 11 x 10-node box: neither is a multiple of the 8 x 8 node tile."""
+@pytest.mark.parametrize("shape,nfreq", [(None, 8), ((13, 10, 9), 8), (None, 4)])
+def test_fused_grid_preconditioner(O, M, c1_model, shape, nfreq, monkeypatch):
+    """A batch of 8 or 16 systems on a box mesh in the grid order runs the preconditioner as one
+    fused kernel (node and edge passes together, help in LDS, a alternating between two
+    buffers). Against the node and edge kernels (MTCOCR_PC_SPLIT=1) on the same batch, whose
+    arithmetic per element it shares (only the dot products' partial sums group differently):
+    x after three iterations within ten times the split path's own rounding spread (the same
+    path with another SpMV layout); solved, FP64 iterations within 2% + 2 and x within 1e-5, mixed
+    true residuals below 1e-8; and the oracle on two systems. Config 1's box (25^3 nodes) and a
+    14 x 11 x 10-node box: neither is a multiple of the 8 x 4 node tile."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
 
     m = c1_model if shape is None else M.random_model(*shape, seed=7, h=100.0, n_pad=2, n_air=3)
     ma = M.mesh_arrays(m)
-    say(f"fused grid preconditioner on {m.shape_cells} cells, 16 systems")
-    freqs = M.frequencies(1e-2, 1e2, 8)
+    say(f"fused grid preconditioner on {m.shape_cells} cells, {2 * nfreq} systems")
+    freqs = M.frequencies(1e-2, 1e2, nfreq)
     cases = [(f, a) for f in freqs for a in (1, 0)]
     p, kd, kg, sd, sg = M.assemble_ks(m)
     prs = np.stack([M.assemble_rhs(m, f, a) for f, a in cases])
@@ -215,9 +215,10 @@ def test_fused_grid_preconditioner(O, M, c1_model, shape, monkeypatch):
                 runs[split, prec] = B.solve_batch(om, prs, eps=1e-8, precision=prec)
             runs[split, "3 iterations"] = B.solve_batch(om, prs, eps=1e-8, maxiter=3)
             if split == "1":
-                # the same path with the other stencil SpMV kernel: the partial sums of (z, s)
-                # group differently, which sets the rounding spread of x after 3 iterations
-                B.set_launch(spmv_layout="stencil")
+                # the same path with another SpMV kernel: the partial sums of (z, s) group
+                # differently, which sets the rounding spread of x after 3 iterations
+                cur = B.get_launch()["spmv_layout"]
+                B.set_launch(spmv_layout="ell" if cur != "ell" else "rowpack")
                 runs["1b", "3 iterations"] = B.solve_batch(om, prs, eps=1e-8, maxiter=3)
     # after three iterations x depends on w through s = s - alpha w: a wrong w would move it by
     # O(1); the fused path stays within the rounding spread of the split path's own kernels
@@ -226,7 +227,7 @@ def test_fused_grid_preconditioner(O, M, c1_model, shape, monkeypatch):
     x3b, s3b = runs["1b", "3 iterations"]
     d3 = max(rel(x3f[k], x3s[k]) for k in range(len(cases)))
     dref = max(rel(x3b[k], x3s[k]) for k in range(len(cases)))
-    say(f"  after 3 iterations: x max rel diff {d3:.2e} (split path, two SpMV kernels: "
+    say(f"  after 3 iterations: x max rel diff {d3:.2e} (split path, two SpMV layouts: "
         f"{dref:.2e})")
     assert [q["iter"] for q in s3f] == [q["iter"] for q in s3s] == [q["iter"] for q in s3b]
     assert d3 <= max(10 * dref, 1e-12)
@@ -247,7 +248,7 @@ def test_fused_grid_preconditioner(O, M, c1_model, shape, monkeypatch):
     # against the oracle: the small box is far less well conditioned (its rounding spread after
     # three iterations is 1e5 times config 1's), so its x is held looser
     xf, sf = runs["0", "fp64"]
-    for k in (0, 15):
+    for k in (0, len(cases) - 1):
         f, a = cases[k]
         xo, so, _ = O.pcocr(M.assemble(m, f, a), ma, eps=1e-8)
         assert abs(sf[k]["iter"] - so["iter"]) <= 2 + 0.02 * so["iter"]
