@@ -13,10 +13,12 @@ synchronisations). Reported beside it:
   fp64_single    the plain FP64 GPU solver, one system at a time (the comparison baseline)
   mixed_batched  FP32 inner COCR + FP64 refinement on the true residual, FP64 finish for the
                  systems whose refinement stalls
-The roofline is that of the dominant kernel (the batched FP64 SpMV), timed with HIP events on
-the solver's stream. The CPU baseline is the oracle (FP64 CPU restatement of the reference's
-COCR_FP) on one system of the same workload for a bounded number of iterations, extrapolated
-with the iteration count the FP64 GPU solver needed for that system.
+The roofline is that of the dominant kernel per iteration (the longer of the batched FP64
+update pass and the batched FP64 SpMV), timed with HIP events on the solver's stream; the SpMV,
+the update pass and the x/p pass (every fourth iteration) are each reported with their HBM
+traffic from rocprofv3 counters. The CPU baseline is the oracle (FP64 CPU restatement of the
+reference's COCR_FP) run to the stop test on one system of the same workload, on the host's
+cores.
 """
 from __future__ import annotations
 
@@ -190,11 +192,11 @@ SPMV_KERNELS = {"ell": "k_spmv", "rowpack": "k_spmv_rg", "rowblock": "k_spmv_blk
 
 
 def measure_traffic(args, sp):
-    """HBM bytes per launch of the batched FP64 SpMV and of the fused update pass: two
-    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over this script's --kernels-only
+    """HBM bytes per launch of the batched FP64 SpMV, the fused update pass and the x/p pass:
+    two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over this script's --kernels-only
     mode, each scaled by what the counter reads for the 1 GiB copy kernel k_copy (known read and
-    write bytes). Returns {"spmv": {...}, "update": {...}}; None when rocprofv3 is unavailable or
-    a pass fails."""
+    write bytes). Returns {"spmv": {...}, "update": {...}, "flush": {...}}; None when rocprofv3
+    is unavailable or a pass fails."""
     import csv
     import re
     import shutil
@@ -206,8 +208,9 @@ def measure_traffic(args, sp):
     if exe is None:
         return None
     pats = {"spmv": re.compile(r"k_spmv(_rg|_blk|_stc|_stl)?<double, %d, 0>" % sp),
-            "update": re.compile(r"k_xr_fused<double, %d>" % sp)}
-    res = {"spmv": {}, "update": {}}
+            "update": re.compile(r"k_xr_fused<double, %d, true>" % sp),
+            "flush": re.compile(r"k_flush_xp<%d>" % sp)}
+    res = {k: {} for k in pats}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--output-format", "csv", "--",
@@ -320,10 +323,12 @@ def main():
     ap.add_argument("--adjoint-cells", type=int, default=4,
                     help="cells of the config-4 adjoint vs finite-difference check (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 --pmc passes that measure SpMV HBM traffic")
+                    help="skip the two rocprofv3 --pmc passes that measure the kernels' HBM "
+                         "traffic")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="profiling mode: autotune, then only the batched SpMV (--spmv-reps "
-                         "launches) and the 1 GiB copy calibration kernel; prints one line")
+                    help="profiling mode: autotune, then only the batched SpMV, update pass and "
+                         "x/p pass (--spmv-reps launches each) and the 1 GiB copy calibration "
+                         "kernel; prints one line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -364,11 +369,14 @@ def main():
         S.set_rhs(prs, om)
         ms, by = S.bench_spmv("fp64", args.spmv_reps)
         ums, uby = S.bench_update("fp64", args.spmv_reps)
+        fms, fby = S.bench_flush(args.spmv_reps)
         cms = S.bench_copy(1 << 30, args.spmv_reps)
         print(json.dumps({"spmv_fp64_batched_ms": ms, "spmv_algorithmic_bytes": by,
                           "spmv_gbs": by / (ms * 1e-3) / 1e9,
                           "update_fp64_batched_ms": ums, "update_algorithmic_bytes": uby,
-                          "update_gbs": uby / (ums * 1e-3) / 1e9, "copy_bytes": 1 << 30,
+                          "update_gbs": uby / (ums * 1e-3) / 1e9,
+                          "xp_fp64_batched_ms": fms, "xp_algorithmic_bytes": fby,
+                          "xp_gbs": fby / (fms * 1e-3) / 1e9, "copy_bytes": 1 << 30,
                           "copy_ms": cms, "copy_gbs": 2 * (1 << 30) / (cms * 1e-3) / 1e9,
                           "launch": tune}), flush=True)
         S.close()
@@ -440,6 +448,8 @@ def main():
     upd_ms, upd_bytes = S.bench_update("fp64", args.spmv_reps)
     achu = gbs(upd_bytes, upd_ms)
     log(f"update pass fp64 x{nsys}: {upd_ms:.3f} ms {achu:.0f} GB/s")
+    xp_ms, xp_bytes = S.bench_flush(args.spmv_reps)
+    log(f"x/p pass fp64 x{nsys}: {xp_ms:.3f} ms {gbs(xp_bytes, xp_ms):.0f} GB/s")
     S.close()
 
     # ---- impedance agreement at a 5x5 surface grid (200 m spacing) ---------------------------
@@ -533,8 +543,10 @@ def main():
     spmv_line = kernel_line(f"{SPMV_KERNELS[tune['batched_fp64']['spmv_layout']]}"
                             f"<double,{spb},0> (batched FP64 SpMV)", spmv_bytes, spmv_ms,
                             traffic["spmv"] if traffic else None)
-    upd_line = kernel_line(f"k_xr_fused<double,{spb}> (batched FP64 update pass)", upd_bytes,
-                           upd_ms, traffic["update"] if traffic else None)
+    upd_line = kernel_line(f"k_xr_fused<double,{spb},true> (batched FP64 update pass)",
+                           upd_bytes, upd_ms, traffic["update"] if traffic else None)
+    xp_line = kernel_line(f"k_flush_xp<{spb}> (batched FP64 x/p pass, every 4th iteration)",
+                          xp_bytes, xp_ms, traffic["flush"] if traffic else None)
     # the dominant kernel: both run once per iteration, the longer one
     roof = upd_line if upd_ms > spmv_ms else spmv_line
     out = {
@@ -559,6 +571,7 @@ def main():
         "roofline": roof,
         "spmv_fp64_batched": spmv_line,
         "update_fp64_batched": upd_line,
+        "xp_fp64_batched": xp_line,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "oracle_vs_gpu_full_size": oracle_cmp,

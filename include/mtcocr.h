@@ -16,9 +16,10 @@
  *   mtcocr_get_solution       no single counterpart: they batch the reference's per-(f, pol)
  *                             COCR_FP processes (Code/CalcStarter/main.cpp:291-325) over one
  *                             shared sparsity, A_s = K + i*omega_s*S.
- *   mtcocr_bench_spmv, mtcocr_bench_copy
- *                             no counterpart (measurement hooks: the SpMV kernel, and a plain
- *                             copy kernel that calibrates the HBM byte counters).
+ *   mtcocr_bench_spmv, mtcocr_bench_update, mtcocr_bench_flush, mtcocr_bench_copy
+ *                             no counterpart (measurement hooks: the SpMV kernel, the update
+ *                             pass, the x/p pass, and a plain copy kernel that calibrates the
+ *                             HBM byte counters).
  *   mtcocr_set_launch, mtcocr_get_launch, mtcocr_autotune
  *                             no counterpart (launch shapes of the GPU kernels; the reference
  *                             is a CPU OpenMP solver with no tunable launch).
@@ -245,13 +246,21 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
 
 /* Kernel-level timing hook, no counterpart in the reference (its per-iteration vector updates,
  * pcocr_rci.cpp:135-154 and MRS.cpp:85-153, run as separate loops): average time (ms) of the
- * fused update pass (k_xr_fused: the MRS blend, p = s + beta p, x += alpha p, r -= alpha a,
- * s -= alpha w and their partials) over the systems of the current batch, `reps` launches on
- * filled work vectors, and the algorithmic bytes per launch (each stream read or written once
- * per system and edge: 12 streams in FP64, 14 with the FP32 vectors of the mixed solver, which
- * also smooth x). */
+ * fused update pass (k_xr_fused: the MRS blend, r -= alpha a, s -= alpha w and their partials;
+ * in the FP32-vector inner solve of the mixed solver also p = s + beta p, x += alpha p and the
+ * smoothed x) over the systems of the current batch, `reps` launches on filled work vectors,
+ * and the algorithmic bytes per launch (each stream read or written once per system and edge:
+ * 8 streams in FP64, whose x and p are updated by k_flush_xp every fourth iteration; 14 with
+ * the FP32 vectors). */
 int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_ms,
                         double* bytes_per_launch);
+
+/* Kernel-level timing hook, no counterpart in the reference (pcocr_rci.cpp:140-142,164 update
+ * p and x every iteration): average time (ms) of the FP64 solver's x/p pass (k_flush_xp: four
+ * iterations of p = s + beta p, x += alpha p from a ring of four s vectors, run every fourth
+ * iteration) over the systems of the current batch, `reps` launches, and its algorithmic bytes
+ * per launch (x, p and four s read, x and p written: 8 streams per system and edge). */
+int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes_per_launch);
 
 /* Calibration hook: average time (ms) of a plain device-to-device copy kernel (k_copy, 16-byte
  * elements) over nbytes (a multiple of 16), `reps` launches after one warm-up. Its HBM byte

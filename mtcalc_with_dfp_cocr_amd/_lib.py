@@ -104,6 +104,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mtcocr_bench_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mtcocr_bench_flush": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "mtcocr_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f64p, f64p]),
     "mtcocr_apply_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                        C.c_void_p]),

@@ -169,6 +169,13 @@ class Solver:
                                           C.byref(ms), C.byref(by)), "mtcocr_bench_update")
         return ms.value, by.value
 
+    def bench_flush(self, reps: int = 20):
+        """Average ms and algorithmic bytes of the FP64 solver's x/p pass (every 4th iteration)."""
+        ms, by = C.c_double(), C.c_double()
+        check(self._L.mtcocr_bench_flush(self._h, int(reps), C.byref(ms), C.byref(by)),
+              "mtcocr_bench_flush")
+        return ms.value, by.value
+
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         """Average ms of the plain copy kernel over nbytes (HBM counter calibration)."""
         ms = C.c_double()

@@ -87,6 +87,9 @@ struct SysScalars {
   int outer_status, outer, last_improve, stall_window;
   int fallback, fallback_iter, rollback, halt;
   int last, pad1, pad2, pad3;  // last: the iteration whose x is the answer (buffer last % 2)
+  // alpha (re, im) and beta (re, im) of iteration k at position (k - 1) % 8 of a chunk of 8:
+  // the FP64 solver applies the x/p recurrence four iterations at a time from these
+  double hab[8][4];
 };
 
 template <class T>
@@ -1957,14 +1960,20 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
 // re-reading x and r, and forming p here saves the preconditioner pass reading s and p: 8 vector
 // reads and 6 writes instead of 13 and 6. With ym null (the FP64 solver, which returns x
 // unsmoothed) the ym blend is skipped: 7 reads and 5 writes.
-template <class T, int SP>
+//
+// LAZY (the FP64 solver): x and p are left out (k_flush_xp applies their recurrence four
+// iterations at a time), and s_k = s_{k-1} - alpha w goes to the next buffer of a ring
+// (s_src -> s_vec) instead of overwriting s_{k-1}, which the x/p recurrence still needs. The
+// pass then moves 8 vectors instead of 12. Eager mode passes s_src == s_vec (each element is
+// read before it is written, by the same thread).
+template <class T, int SP, bool LAZY = false>
 __global__ __launch_bounds__(kBlock) void k_xr_fused(
     const typename Vec2<T>::type* __restrict__ xin, typename Vec2<T>::type* __restrict__ xout,
     typename Vec2<T>::type* __restrict__ p, typename Vec2<T>::type* __restrict__ r,
-    const typename Vec2<T>::type* __restrict__ a, typename Vec2<T>::type* __restrict__ s_vec,
-    const typename Vec2<T>::type* __restrict__ w, typename Vec2<T>::type* __restrict__ ym,
-    typename Vec2<T>::type* __restrict__ sm, const SysScalars* __restrict__ sc,
-    const int* __restrict__ nrun, double* partials, int nb) {
+    const typename Vec2<T>::type* __restrict__ a, const typename Vec2<T>::type* s_src,
+    typename Vec2<T>::type* s_vec, const typename Vec2<T>::type* __restrict__ w,
+    typename Vec2<T>::type* __restrict__ ym, typename Vec2<T>::type* __restrict__ sm,
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, double* partials, int nb) {
   using T2 = typename Vec2<T>::type;
   if (*nrun == 0) return;
   const long long N = (long long)nb * SP;
@@ -1996,12 +2005,12 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
       T2 x, r, sm, s, p, a, w;
     };
     auto load = [&](long long t, Elem& E) {
-      E.x = ld_nt(xsrc + t);
+      if (!LAZY) E.x = ld_nt(xsrc + t);
       E.r = ld_nt(r + t);
       E.sm = ld_nt(sm + t);
       if (step2) {
-        E.s = ld_nt(s_vec + t);
-        E.p = ld_nt(p + t);
+        E.s = ld_nt(s_src + t);
+        if (!LAZY) E.p = ld_nt(p + t);
         E.a = ld_nt(a + t);
         E.w = ld_nt(w + t);
       }
@@ -2009,7 +2018,7 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
     auto update = [&](long long t, Elem& E) {
       T2 smv = E.sm;
       if (mode == MRS_BLEND) {
-        if (ym) {
+        if (!LAZY && ym) {
           const T2 y0 = ym[t];
           T2 yv = y0;
           yv.x += wm * (E.x.x - yv.x);
@@ -2027,19 +2036,21 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
         st_nt(sm + t, smv);
         v[2] += double(smv.x) * double(smv.x) + double(smv.y) * double(smv.y);
       } else if (mode == MRS_COPY) {
-        if (ym) ym[t] = E.x;
+        if (!LAZY && ym) ym[t] = E.x;
         smv = E.r;
         st_nt(sm + t, smv);
         v[2] += double(E.r.x) * double(E.r.x) + double(E.r.y) * double(E.r.y);
       }
       if (!step2) return;
-      const T2 pn = caxpy(E.s, E.p, br, bi);
-      // x of a lane not running is stored exactly as loaded (dead inputs may hold anything)
-      const T2 xn = run ? caxpy(E.x, pn, ar, ai) : E.x;
+      if (!LAZY) {
+        const T2 pn = caxpy(E.s, E.p, br, bi);
+        // x of a lane not running is stored exactly as loaded (dead inputs may hold anything)
+        const T2 xn = run ? caxpy(E.x, pn, ar, ai) : E.x;
+        st_nt(p + t, pn);
+        st_nt(xout + t, xn);
+      }
       const T2 rn = caxpy(E.r, E.a, -ar, -ai);
       st_nt(s_vec + t, caxpy(E.s, E.w, -ar, -ai));
-      st_nt(p + t, pn);
-      st_nt(xout + t, xn);
       st_nt(r + t, rn);
       const T ur = rn.x - smv.x, ui = rn.y - smv.y;
       v[0] += double(smv.x) * double(ur) + double(smv.y) * double(ui);
@@ -2064,6 +2075,85 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
   block_partials<SP, 3>(v, partials);
 }
 
+
+// The x/p recurrence of four COCR iterations k-3..k in one pass (pcocr_rci.cpp:140-142,164:
+// p_j = s_{j-1} + beta_j p_{j-1}, x_j = x_{j-1} + alpha_j p_j), for the FP64 solver whose
+// update pass leaves x and p out (k_xr_fused<.., true>). Nothing else in an iteration reads x
+// or p, so they can lag; the update pass keeps s_{k-4}..s_{k-1} in a ring of eight buffers
+// (s_i below is s_{k-4+i}) and the scalar kernels record each iteration's alpha and beta
+// (SysScalars::hab, position (j - 1) % 8 = c0 + i). The arithmetic is that of the eager update
+// pass, in the same order, so x and p carry the same bits.
+//
+// Runs after s_mrs_stop of iteration k (k = nrun[1], a multiple of 4), when every stop up to
+// iteration k - 1 is known: a system that stopped at iteration `last` takes the steps j <= last
+// only, so x then holds its answer, x_last. A system still running takes all four; should it
+// stop at iteration k, x_k is its answer. Lanes with nothing to apply store x as loaded (their
+// dead p and s may hold anything). x and p are updated in place: each element is read and
+// written by one thread. It runs even when no system is running any more (at most twice per
+// solve), since the last stops of a batch complete their x here.
+template <int SP>
+__global__ __launch_bounds__(kBlock) void k_flush_xp(
+    double2* __restrict__ x, double2* __restrict__ p, const double2* __restrict__ s0,
+    const double2* __restrict__ s1, const double2* __restrict__ s2,
+    const double2* __restrict__ s3, const SysScalars* __restrict__ sc,
+    const int* __restrict__ nrun, int c0, int nb) {
+  const long long N = (long long)nb * SP;
+  const long long ntiles = (N + kBlock - 1) / kBlock;
+  TileWalk tw(ntiles);
+  const int s = int(threadIdx.x % SP);  // kBlock is a multiple of SP: one system per thread
+  const int k = nrun[1];
+  const bool run = sc[s].status == ST_RUNNING;
+  const int last = sc[s].last;
+  bool use[4];
+  double ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = k - 3 + i;
+    use[i] = j >= 1 && (run || j <= last);
+    const double* hq = sc[s].hab[(c0 + i) & 7];
+    ar[i] = hq[0];
+    ai[i] = hq[1];
+    br[i] = hq[2];
+    bi[i] = hq[3];
+  }
+  struct Elem {
+    double2 x, p, s[4];
+  };
+  auto load = [&](long long t, Elem& E) {
+    E.x = ld_nt(x + t);
+    E.p = ld_nt(p + t);
+    E.s[0] = ld_nt(s0 + t);
+    E.s[1] = ld_nt(s1 + t);
+    E.s[2] = ld_nt(s2 + t);
+    E.s[3] = ld_nt(s3 + t);
+  };
+  auto apply = [&](long long t, Elem& E) {
+    double2 xv = E.x, pv = E.p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double2 pn = caxpy(E.s[i], pv, br[i], bi[i]);
+      const double2 xn = caxpy(xv, pn, ar[i], ai[i]);
+      // selects, not branches: every lane of a wave keeps issuing the same memory instructions
+      pv.x = use[i] ? pn.x : pv.x;
+      pv.y = use[i] ? pn.y : pv.y;
+      xv.x = use[i] ? xn.x : xv.x;
+      xv.y = use[i] ? xn.y : xv.y;
+    }
+    st_nt(x + t, xv);
+    st_nt(p + t, pv);
+  };
+  for (long long tile = tw.first; tile < tw.last; tile += 2 * tw.step) {
+    const long long t0 = tile * kBlock + threadIdx.x;
+    const long long t1 = (tile + tw.step) * kBlock + threadIdx.x;
+    const bool ok0 = t0 < N;
+    const bool ok1 = tile + tw.step < tw.last && t1 < N;
+    Elem e0, e1;
+    if (ok0) load(t0, e0);
+    if (ok1) load(t1, e1);
+    if (ok0) apply(t0, e0);
+    if (ok1) apply(t1, e1);
+  }
+}
 
 // ---- batch compaction: narrowing the batch to the systems still running -------------------
 
@@ -2149,6 +2239,11 @@ __global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, in
       cdiv(sc[s].zs_re, sc[s].zs_im, awr, awi, qr, qi);
       sc[s].al_re = qr;
       sc[s].al_im = qi;
+      double* hq = sc[s].hab[(sc[s].iter - 1) & 7];
+      hq[0] = qr;
+      hq[1] = qi;
+      hq[2] = sc[s].be_re;
+      hq[3] = sc[s].be_im;
     }
   }
   __syncthreads();
@@ -2253,7 +2348,12 @@ __global__ __launch_bounds__(kScalarThreads) void s_beta(const double* part, int
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) count_running(sc, SP, nrun);
+  if (threadIdx.x == 0) {
+    count_running(sc, SP, nrun);
+    // nrun[1]: the batch's iteration number (that of every running system), which
+    // k_flush_xp compares with the iteration of a system that has stopped
+    nrun[1] = FIRST ? 1 : nrun[1] + 1;
+  }
 }
 
 // First stage of the partial-sum reduction for wide grids (more than kFoldAbove partial
@@ -2515,6 +2615,7 @@ struct mtcocr_handle {
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
   DevBuf va2;  // the second buffer of a, alternating with va, for k_pc_grid
+  DevBuf vsr[7];  // s buffers 1..7 of the FP64 solver's ring (buffer 0 is vs)
   // the same vectors for a narrowed batch (swapped with the above by compaction)
   DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
   DevBuf cmap;  // slot maps of compaction / scatter
@@ -2566,9 +2667,10 @@ struct mtcocr_handle {
                            &gval32, &gtval32, &omega, &scal, &nrun, &nrun2, &part, &part2, &b64, &x64,
                            &r64, &vr, &vp, &vs, &vz, &va, &vw, &vym, &vsm, &vhelp, &vdn, &vx2, &cvr, &cvp, &cvs, &cvz, &cva, &cvw, &cvym,
                            &cvsm, &cvx, &cvx2, &cmap,
-                           &vx, &vb, &sg_ce, &sg_xyz, &sg_coef, &sg_u, &sg_v, &sg_out};
+                           &vx, &vb, &sg_ce, &sg_xyz, &sg_coef, &sg_u, &sg_v, &sg_out, &va2};
     int64_t t = 0;
     for (auto* b : all) t += (int64_t)b->bytes;
+    for (const DevBuf& b : vsr) t += (int64_t)b.bytes;
     return t;
   }
 };
@@ -2857,6 +2959,10 @@ struct Engine {
   double *part, *part2;
   const double* omega;
   int nb, m, grid_s, grid_e, grid_n, grid_u, layout;
+  // the FP64 solver's ring of s buffers (s_k in sr[k % 8], sr[0] = s): its update pass leaves
+  // x and p to k_flush_xp, which reads s_{k-4}..s_{k-1} every fourth iteration
+  static constexpr bool kLazy = std::is_same<T, double>::value;
+  T2* sr[8];
   // the fused grid preconditioner (k_pc_grid): used, its view and grid
   bool use_pg = false;
   PgView pg;
@@ -2895,6 +3001,8 @@ struct Engine {
     grid_n = grid_for((long long)m * SP, h->cap_node[q][k]);
     layout = h->spmv_layout[q][k];
     a2 = h->va2.as<T2>();
+    sr[0] = s;
+    for (int i = 1; i < 8; ++i) sr[i] = kLazy ? h->vsr[i - 1].as<T2>() : s;
     use_pg = pg_applies(h, SP, sizeof(T2));
     if (use_pg) {
       pg = pg_view(h, h->pg_kc[q]);
@@ -2952,9 +3060,11 @@ struct Engine {
     s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
 
-  // one COCR iteration (stages 5-8); `odd` = the iteration's index is odd (it reads x from
-  // buffer 0 and writes buffer 1), which alternates within a captured chunk of iterations
-  void iteration(hipStream_t st, int maxiter, bool odd) {
+  // one COCR iteration (stages 5-8) at position c = 0..7 of a chunk of 8 (iteration k with
+  // c = (k - 1) % 8). `odd` = k is odd: the eager update pass then reads x from buffer 0 and
+  // writes buffer 1, and k_pc_grid reads a and writes a2.
+  void iteration(hipStream_t st, int maxiter, int c) {
+    const bool odd = (c & 1) == 0;
     int nr;
     const double* pp;
     T2* acur = a;  // a of this iteration after the preconditioner pass
@@ -2973,11 +3083,27 @@ struct Engine {
       pp = partials<SP * 2>(st, grid_e, nr);
     }
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, acur, s, w,
-                                                 ym_or_null(), sm, sc, nrun, part, nb);
+    T2* s_next = s;  // s_k
+    if constexpr (kLazy) {
+      s_next = sr[(c + 1) & 7];
+      k_xr_fused<T, SP, true><<<grid_u, kBlock, 0, st>>>(nullptr, nullptr, nullptr, r, acur,
+                                                         sr[c & 7], s_next, w, nullptr, sm, sc,
+                                                         nrun, part, nb);
+    } else {
+      k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, acur, s, s,
+                                                   w, ym_or_null(), sm, sc, nrun, part, nb);
+    }
     pp = partials<SP * 3>(st, grid_u, nr);
     s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
-    spmv<0>(st, s, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
+    if constexpr (kLazy) {
+      // every fourth iteration (k = 4, 8, ...: positions 3 and 7), once the stops up to k - 1
+      // are known: x and p of iterations k-3..k from s_{k-4}..s_{k-1}
+      if ((c & 3) == 3)
+        k_flush_xp<SP><<<grid_u, kBlock, 0, st>>>(x, p, sr[(c - 3) & 7], sr[(c - 2) & 7],
+                                                  sr[(c - 1) & 7], sr[c & 7], sc, nrun,
+                                                  (c - 3) & 7, nb);
+    }
+    spmv<0>(st, s_next, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     pp = partials<SP * 2>(st, grid_s, nr);
     s_beta<SP, false><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
@@ -3003,6 +3129,10 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
   // FP64 batches run in groups of 16)
   if (sp >= 8 && h->pg_ok && (rc = ensure(h, h->va2, (size_t)h->nb * std::min(sp, 16) * es)))
     return rc;
+  // the FP64 solver's ring of s buffers (Engine::sr)
+  if (prec == MTCOCR_PREC_FP64)
+    for (DevBuf& b : h->vsr)
+      if ((rc = ensure(h, b, nv))) return rc;
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
   if ((rc = ensure(h, h->vdn, nm))) return rc;
   h->work_prec = prec;
@@ -3020,6 +3150,8 @@ template <class T, int SP>
 int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_below = 0) {
   hipStream_t st = h->stream;
   const int prec = std::is_same<T, double>::value ? 0 : 1;
+  // 8 iterations per graph: the FP64 solver's ring of s buffers and its x/p flushes (at
+  // positions 3 and 7) are laid out for chunks of 8 that start at iterations 1, 9, 17, ...
   const int chunk = 8;
   if (use_graphs()) {
     if (!(h->graph && h->graph_prec == prec && h->graph_sp == SP && h->graph_chunk == chunk &&
@@ -3030,7 +3162,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_bel
       }
       hipGraph_t g;
       HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, (c & 1) == 0);
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, c);
       HIPC(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -3059,7 +3191,7 @@ int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_bel
     if (use_graphs()) {
       HIPC(hipGraphLaunch(h->graph, st));
     } else {
-      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, (c & 1) == 0);
+      for (int c = 0; c < chunk; ++c) E.iteration(st, maxiter, c);
     }
     HIPC(hipGetLastError());
     launched += chunk;
@@ -3124,7 +3256,8 @@ int fp64_phase(mtcocr_handle* h, int maxiter, const std::vector<int>& orig,
   const long long nv = (long long)h->nb * SP;
   const int gv = (int)std::min<long long>(4096, (nv + 255) / 256);
   HIPC(hipMemcpyAsync(h->cmap.p, sel.data(), sizeof(int) * SP, hipMemcpyHostToDevice, st));
-  k_scatter_x<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), R.sp0, E.x, E.x2, E.sc,
+  // x holds each finished system's answer (k_flush_xp has applied its steps up to `last`)
+  k_scatter_x<SP><<<gv, 256, 0, st>>>(h->x64.as<double2>(), R.sp0, E.x, E.x, E.sc,
                                       h->cmap.as<int>(), h->nb);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(st));
@@ -3871,13 +4004,21 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
       using T = decltype(tag);
       using T2 = typename Vec2<T>::type;
       Engine<T, SP> E(h);
-      for (T2* v : {E.x, E.x2, E.p, E.r, E.a, E.s, E.w, E.ym, E.sm}) k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+      for (T2* v : {E.x, E.x2, E.p, E.r, E.a, E.s, E.w, E.ym, E.sm, E.sr[1]})
+        k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
       T2* ym = E.ym_or_null();
-      streams = ym ? 14 : 12;
+      // the pass as the solver runs it: FP64 without x and p (8 vectors: r, sm, s, a, w read;
+      // r, sm and the next s written), FP32 vectors with x, p and the smoothed iterate
+      streams = Engine<T, SP>::kLazy ? 8 : ym ? 14 : 12;
       auto launch = [&](int i) {
-        k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(i & 1 ? E.x2 : E.x, i & 1 ? E.x : E.x2,
-                                                       E.p, E.r, E.a, E.s, E.w, ym, E.sm, E.sc,
-                                                       E.nrun, E.part, E.nb);
+        if constexpr (Engine<T, SP>::kLazy)
+          k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
+              nullptr, nullptr, nullptr, E.r, E.a, i & 1 ? E.sr[1] : E.s, i & 1 ? E.s : E.sr[1],
+              E.w, nullptr, E.sm, E.sc, E.nrun, E.part, E.nb);
+        else
+          k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(i & 1 ? E.x2 : E.x, i & 1 ? E.x : E.x2,
+                                                         E.p, E.r, E.a, E.s, E.s, E.w, ym, E.sm,
+                                                         E.sc, E.nrun, E.part, E.nb);
       };
       launch(1);
       HIPC(hipEventRecord(h->ev0, st));
@@ -3897,6 +4038,47 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
   // algorithmic bytes: each stream read or written once per system and edge
   const double ev = (precision == MTCOCR_PREC_FP64) ? 16.0 : 8.0;
   *bytes_per_launch = double(streams) * ev * double(h->nb) * double(h->nsys);
+  return 0;
+}
+
+int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes_per_launch) {
+  if (!h || reps < 1 || !avg_ms || !bytes_per_launch) return MTCOCR_ERR_ARG;
+  if (!h->have_matrix || h->nsys < 1) return MTCOCR_ERR_STATE;
+  HIPC(hipSetDevice(h->device));
+  const int sp = h->sp;
+  int rc = alloc_work(h, MTCOCR_PREC_FP64, sp);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, h->nsys, sp, 1e-8);  // running systems: every lane takes all four steps
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  const int kn[2] = {1, 4};  // [0] running count, [1] the iteration a flush ends
+  HIPC(hipMemcpyAsync(h->nrun.p, kn, sizeof(kn), hipMemcpyHostToDevice, st));
+  const long long nv = (long long)h->nb * sp;
+  const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
+  float ms = 0.f;
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    Engine<double, SP> E(h);
+    for (double2* v : {E.x, E.p, E.sr[0], E.sr[1], E.sr[2], E.sr[3]})
+      k_fill<double2><<<gf, 256, 0, st>>>(v, nv);
+    auto launch = [&]() {
+      k_flush_xp<SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, E.sr[0], E.sr[1], E.sr[2], E.sr[3],
+                                                  E.sc, E.nrun, 0, E.nb);
+    };
+    launch();
+    HIPC(hipEventRecord(h->ev0, st));
+    for (int i = 0; i < reps; ++i) launch();
+    HIPC(hipEventRecord(h->ev1, st));
+    HIPC(hipGetLastError());
+    HIPC(hipEventSynchronize(h->ev1));
+    HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    return 0;
+  });
+  if (rc) return rc;
+  *avg_ms = double(ms) / reps;
+  // x, p and four s read, x and p written, once per system and edge
+  *bytes_per_launch = 8.0 * 16.0 * double(h->nb) * double(h->nsys);
   return 0;
 }
 
@@ -4041,7 +4223,7 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
         const long long nv = (long long)E.nb * SP, nm = (long long)std::max(E.m, 1) * SP;
         const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
         const int gm = (int)std::min<long long>(2048, (nm + 255) / 256);
-        for (T2* v : {E.s, E.z, E.a, E.p, E.w, E.x, E.x2, E.r, E.ym, E.sm})
+        for (T2* v : {E.s, E.z, E.a, E.p, E.w, E.x, E.x2, E.r, E.ym, E.sm, E.sr[1]})
           k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
         k_fill<T2><<<gm, 256, 0, st>>>(E.help, nm);
         E.setup_diagonals(st);
@@ -4108,7 +4290,9 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 &per_cu, reinterpret_cast<const void*>(k_pc_edge_iter<T, SP>), kBlock, 0));
           } else {
             HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(k_xr_fused<T, SP>), kBlock, 0));
+                &per_cu,
+                reinterpret_cast<const void*>(k_xr_fused<T, SP, Engine<T, SP>::kLazy>),
+                kBlock, 0));
           }
           std::vector<int> cands(std::begin(kCands), std::end(kCands));
           const int resident = round_up(std::max(1, per_cu * h->n_cu), kXcd);
@@ -4149,9 +4333,15 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
                 const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
                 s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               } else {
-                k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.x2, E.p, E.r, E.a, E.s,
-                                                               E.w, E.ym_or_null(), E.sm, E.sc,
-                                                               E.nrun, E.part, E.nb);
+                // the pass as the solver runs it (FP64: without x and p, s into the ring)
+                if constexpr (Engine<T, SP>::kLazy)
+                  k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
+                      nullptr, nullptr, nullptr, E.r, E.a, E.s, E.sr[1], E.w, nullptr, E.sm,
+                      E.sc, E.nrun, E.part, E.nb);
+                else
+                  k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(
+                      E.x, E.x2, E.p, E.r, E.a, E.s, E.s, E.w, E.ym_or_null(), E.sm, E.sc,
+                      E.nrun, E.part, E.nb);
                 int nr;
                 const double* pp = E.template partials<SP * 3>(st, E.grid_u, nr);
                 s_reduce_probe<SP * 3><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);

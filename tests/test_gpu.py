@@ -618,8 +618,10 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
 
 
 def test_bench_hooks(M, c1_model, c1_mesh):
-    """Kernel timing hooks on a batch of 5 systems (width 8): SpMV and the fused update pass
-    report positive times and their algorithmic bytes; the resident batch is left usable."""
+    """Kernel timing hooks on a batch of 5 systems (width 8): SpMV, the fused update pass and
+    the FP64 x/p pass report positive times and their algorithmic bytes (the FP64 update pass
+    moves 8 vectors, the FP32-vector one 14, the x/p pass 8); the resident batch is left
+    usable."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     from mtcalc_with_dfp_cocr_amd.cocr import Solver
@@ -633,11 +635,13 @@ def test_bench_hooks(M, c1_model, c1_mesh):
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(c1_mesh)
         B.set_rhs(prs, om)
-        for prec, streams, elem in (("fp64", 12, 16), ("mixed", 14, 8)):
+        for prec, streams, elem in (("fp64", 8, 16), ("mixed", 14, 8)):
             ms, by = B.bench_update(prec, 3)
             assert ms > 0 and by == streams * elem * p.nb * len(cases)
             ms, by = B.bench_spmv(prec, 3)
             assert ms > 0 and by > 0
+        ms, by = B.bench_flush(3)
+        assert ms > 0 and by == 8 * 16 * p.nb * len(cases)
         B.set_rhs(prs, om)
         rc, st = B.solve_resident(1e-8, 10000, "fp64")
         assert all(q["req"] == 0 for q in st)
