@@ -365,16 +365,18 @@ __device__ inline typename Vec2<T>::type diag_inv(const DiagA& D, int row, int s
   return o;
 }
 
-// value of A(j, c) = a + i*omega*b in row j of the sliced ELL (0 when absent)
-__device__ inline bool ell_lookup(const EllView<double>& A, int j, int c, double w, double& vr,
-                                  double& vi) {
+// entry (j, c) of the sliced ELL as stored: a and b of a complex-section entry (cplx) or a of a
+// real-section one (b = 0); false when absent
+__device__ inline bool ell_lookup_ab(const EllView<double>& A, int j, int c, double& va,
+                                     double& vb, bool& cplx) {
   const int slice = j >> 6, lane = j & 63;
   const int oc = A.offc[slice], ncs = A.nc[slice];
   for (int q = 0; q < ncs; ++q) {
     const int idx = oc + q * 64 + lane;
     if (A.colc[idx] == c) {
-      vr = A.ac[idx];
-      vi = w * A.bc[idx];
+      va = A.ac[idx];
+      vb = A.bc[idx];
+      cplx = true;
       return true;
     }
   }
@@ -382,8 +384,9 @@ __device__ inline bool ell_lookup(const EllView<double>& A, int j, int c, double
   for (int q = 0; q < nrs; ++q) {
     const int idx = orr + q * 64 + lane;
     if (A.colr[idx] == c) {
-      vr = A.ar[idx];
-      vi = 0.0;
+      va = A.ar[idx];
+      vb = 0.0;
+      cplx = false;
       return true;
     }
   }
@@ -394,16 +397,17 @@ __device__ inline bool ell_lookup(const EllView<double>& A, int j, int c, double
 // in the reference's order (for every edge j of node i ascending: the entries of row j in
 // ascending column order, then times G_ij) without fused multiply-adds. In air cells the real part
 // of (G A G^T)_ii is the rounding residue of G K G^T = 0, so this order is what fixes its value.
+//
+// One thread per node: the node's matrix entries are looked up once and its SP systems' sums are
+// formed from them (up to 16 systems per pass over the entries, each system's additions in
+// exactly the order above), instead of one thread per node and system repeating the lookups.
 template <class T, int SP>
 __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega,
                          typename Vec2<T>::type* out) {
 #pragma clang fp contract(off)
-  const long long n = (long long)G.m * SP;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = int(t / SP);
-    const int s = int(t % SP);
-    const double w = omega[s];
+  constexpr int CH = SP < 16 ? SP : 16;  // systems per pass over the node's entries
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G.m;
+       i += (long long)gridDim.x * blockDim.x) {
     int ecol[kMaxGRow];
     double eval[kMaxGRow];
     int wg = 0;
@@ -414,24 +418,47 @@ __global__ void k_diag_n(EllView<double> A, GView<double> G, const double* omega
       eval[wg] = g;
       ++wg;
     }
-    double dr = 0.0, di = 0.0;
-    for (int kj = 0; kj < wg; ++kj) {
-      const int j = ecol[kj];
-      double sr = 0.0, si = 0.0;
-      for (int kk = 0; kk < wg; ++kk) {  // ascending edge = ascending column of row j
-        double vr, vi;
-        if (ell_lookup(A, j, ecol[kk], w, vr, vi)) {
-          sr += vr * eval[kk];
-          si += vi * eval[kk];
+    for (int s0 = 0; s0 < SP; s0 += CH) {
+      double w[CH], dr[CH], di[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        w[c] = omega[s0 + c];
+        dr[c] = 0.0;
+        di[c] = 0.0;
+      }
+      for (int kj = 0; kj < wg; ++kj) {
+        const int j = ecol[kj];
+        double sr[CH], si[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) sr[c] = si[c] = 0.0;
+        for (int kk = 0; kk < wg; ++kk) {  // ascending edge = ascending column of row j
+          double va, vb;
+          bool cplx;
+          if (ell_lookup_ab(A, j, ecol[kk], va, vb, cplx)) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              const double vi = cplx ? w[c] * vb : 0.0;
+              sr[c] += va * eval[kk];
+              si[c] += vi * eval[kk];
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          dr[c] += sr[c] * eval[kj];
+          di[c] += si[c] * eval[kj];
         }
       }
-      dr += sr * eval[kj];
-      di += si * eval[kj];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        double qr, qi;
+        cdiv(1.0, 0.0, dr[c], di[c], qr, qi);
+        typename Vec2<T>::type o;
+        o.x = T(qr);
+        o.y = T(qi);
+        out[i * SP + s0 + c] = o;
+      }
     }
-    double qr, qi;
-    cdiv(1.0, 0.0, dr, di, qr, qi);
-    out[t].x = T(qr);
-    out[t].y = T(qi);
   }
 }
 
@@ -3038,7 +3065,7 @@ struct Engine {
 
   // D_N^-1 per system (D_A^-1 is evaluated inside the edge kernels)
   void setup_diagonals(hipStream_t st) {
-    const int g2 = std::min(4096, (int)(((long long)m * SP + 255) / 256) + 1);
+    const int g2 = std::min(4096, (int)(((long long)m + 255) / 256) + 1);
     k_diag_n<T, SP><<<g2, 256, 0, st>>>(ell_view<double>(h), g_view<double>(h), omega, dn);
   }
 
