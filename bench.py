@@ -71,8 +71,10 @@ def build_workload(args):
 
 
 def batched_only(m, freqs, args, label):
-    """Batched FP64 solve of all (frequency x polarization) systems of one model, autotuned,
-    pencil-ordered; returns a summary (one warm-up solve, then `steps` timed solves)."""
+    """Batched FP64 solve of all (frequency x polarization) systems of one model, autotuned, in
+    the device row order --order-block sets; returns a summary (warm-up solves, then `steps`
+    timed solves), with the mixed-precision form beside it (unless --no-mixed) and, for a batch
+    of at most 16 systems, the SpMV, update pass and x/p pass rates on it."""
     from mtcalc_with_dfp_cocr_amd import model as M
     from mtcalc_with_dfp_cocr_amd.cocr import Solver
 
@@ -111,12 +113,37 @@ def batched_only(m, freqs, args, label):
             times.append(dt)
         log(f"{label}: {'warmup' if k < args.warmup else 'step'} {dt:.3f}s rc={rc}")
     info = S.info()
+    out = {"model": label, "cells": list(m.shape_cells), "nb": int(p.nb),
+           "systems": len(cases), "s_per_step": float(np.mean(times)),
+           "s_per_system": float(np.mean(times)) / len(cases),
+           "ok": all(q["req"] == 0 for q in st), "iters": [q["iter"] for q in st],
+           "true_rel_res_max": max(q["true_rel_res"] for q in st),
+           "device_bytes": info["device_bytes"], "launch": tune, "host_setup_s": t_host}
+    if not args.no_mixed:
+        # the mixed-precision form (FP32 inner COCR, FP64 refinement and finish), once
+        dt, ms_st = 0.0, []
+        for g in groups:
+            S.set_rhs(prs[g], om[g])
+            d, r, q = timed(S, "mixed", args)
+            dt += d
+            ms_st += q
+        out["mixed"] = {"s_per_system": dt / len(cases), "ok": all(q["req"] == 0 for q in ms_st),
+                        "true_rel_res_max": max(q["true_rel_res"] for q in ms_st),
+                        "fp64_finish": sum(q["flags"] & 1 for q in ms_st),
+                        "device_bytes": S.info()["device_bytes"]}
+        log(f"{label}: mixed {out['mixed']}")
+    if len(groups) == 1 and len(cases) <= 16:
+        # kernel rates on this batch, as the solver launches them (HIP events)
+        S.set_rhs(prs, om)
+        gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+        for name, (ms, by) in (("spmv_fp64_batched", S.bench_spmv("fp64", 10)),
+                               ("update_fp64_batched", S.bench_update("fp64", 10)),
+                               ("xp_fp64_batched", S.bench_flush(10))):
+            out[name] = {"avg_ms": ms, "algorithmic_bytes": by, "achieved_gbs": gbs(by, ms),
+                         "frac": gbs(by, ms) / HBM_PEAK_GBS}
+        log(f"{label}: kernel rates { {k: out[k] for k in out if k.endswith('_batched')} }")
     S.close()
-    step = float(np.mean(times))
-    return {"model": label, "cells": list(m.shape_cells), "nb": int(p.nb),
-            "systems": len(cases), "s_per_step": step, "s_per_system": step / len(cases),
-            "ok": all(q["req"] == 0 for q in st), "iters": [q["iter"] for q in st],
-            "device_bytes": info["device_bytes"], "launch": tune, "host_setup_s": t_host}
+    return out
 
 
 def run_config(args):
