@@ -380,6 +380,43 @@ def test_special_cases(S, M, c1_system, c1_mesh):
         assert st["req"] == -2 and st["iter"] == 1
 
 
+def test_iteration_limit_at_every_xp_position(O, M, c1_model, c1_mesh):
+    """The FP64 solver applies x and p four iterations at a time; a system that halts at its
+    iteration limit returns x of the last iteration whatever its position in that window. A
+    batch of 16 systems (stencil SpMV, fused preconditioner) halted at maxiter = 57..61: each
+    checked system's x against the oracle's x at the same limit, closer to it than to the
+    oracle's x one iteration before or after by a factor of 100."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    limits = (57, 58, 59, 60, 61)
+    gpu = {}
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, ORDER_GRID)
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        for mi in limits:
+            gpu[mi] = B.solve_batch(om, prs, eps=1e-8, maxiter=mi, precision="fp64")
+    for k in (0, 5, 10, 15):
+        f, a = cases[k]
+        s = M.assemble(c1_model, f, a)
+        xo = {mi: O.pcocr(s, c1_mesh, eps=1e-8, maxiter=mi)[0]
+              for mi in range(limits[0] - 1, limits[-1] + 2)}
+        for mi in limits:
+            x, st = gpu[mi]
+            assert st[k]["req"] == -3 and st[k]["iter"] == mi + 1
+            d = rel(x[k], xo[mi])
+            gap = min(rel(xo[mi - 1], xo[mi]), rel(xo[mi + 1], xo[mi]))
+            say(f"case {k} maxiter {mi}: x vs oracle {d:.2e}, one iteration apart {gap:.2e}")
+            assert d < 0.01 * gap
+
+
 def test_pcocr_2x2_folded_mirror(S, M, c1_system, c1_mesh):
     """The reference-signature entry point gives the same x as the handle API (deterministic)."""
     say("pcocr_2x2_folded mirror")
