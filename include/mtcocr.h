@@ -58,6 +58,10 @@ extern "C" {
 #define MTCOCR_ERR_NOTCONV (-14) /* mixed precision: refinement stalled above the tolerance */
 
 #define MTCOCR_FLAG_FP64_FINISH 1 /* mixed: refinement stalled, finished by the FP64 solver */
+/* mixed: the FP64 finish met the stop test at eps/30 yet the true residual (stats.true_rel_res)
+ * stayed at or above eps, the attainable accuracy of the recursively updated residual; the
+ * answer is returned as the FP64 solver's would be (req MTCOCR_OK) */
+#define MTCOCR_FLAG_RES_FLOOR 2
 
 #define MTCOCR_ORDER_GRID (-1) /* mtcocr_set_mesh_order: the grid order of a box mesh */
 
@@ -239,7 +243,8 @@ int mtcocr_get_stencil_band(mtcocr_handle* h, int precision, int* band);
  * and batch width (padded system count). FP64 batches wider than 16 systems run their
  * iterations as groups of 16: they are tuned at width 16, which the outputs then report, while
  * mtcocr_get_launch reports the width of the batch itself (used by the kernel-level hooks).
- * Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the chosen
+ * In FP64 the widths that compaction narrows a batch to (half, a quarter, ... one system) are
+ * tuned too, on the batch's first systems. Outputs (any may be NULL): the chosen caps and layout, and the SpMV time in ms at the chosen
  * shape. Writes only work buffers: resident right-hand sides and solutions stay. */
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms);

@@ -3577,11 +3577,14 @@ int solve_mixed(mtcocr_handle* h, double eps, int maxiter, mtcocr_stats* stats) 
         extra += fs[s].seconds;
         counted = true;
       }
-      if (fs[s].req == MTCOCR_OK && fs[s].true_rel_res >= eps) {
+      if (fs[s].req == MTCOCR_OK && fs[s].true_rel_res >= eps && attempt == 0) {
         o.req = MTCOCR_ERR_NOTCONV;  // unless the next attempt gets there
         any = true;
       } else {
+        // after the second attempt the FP64 stop test (at eps/30) decides, as for the plain
+        // FP64 solver; a true residual still at or above eps is flagged
         o.req = fs[s].req;
+        if (fs[s].req == MTCOCR_OK && fs[s].true_rel_res >= eps) o.flags |= MTCOCR_FLAG_RES_FLOOR;
         fall[s] = 0;
       }
     }
@@ -3709,6 +3712,201 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
   if (h->have_g && h->g_nb != h->nb) h->have_g = false;
   h->nsys = 0;
   h->work_prec = -1;
+  return 0;
+}
+
+// The tuner for one batch width sp (the first min(nsys, sp) systems of the batch on the handle);
+// it stores the chosen shapes in the handle's per-width tables and returns the SpMV time at the
+// chosen shape in *spmv_ms_out.
+int autotune_width(mtcocr_handle* h, int precision, int reps, int sp, double* spmv_ms_out) {
+  int rc = alloc_work(h, precision, sp);
+  if (rc) return rc;
+  hipStream_t st = h->stream;
+  std::vector<SysScalars> hs;
+  set_eps_status(h, hs, std::min(h->nsys, sp), sp, 1e-8);
+  for (auto& q : hs) q.mode = MRS_BLEND;  // the update pass's common step
+  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
+  int one = 1;
+  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  // large grids: the streaming kernels reach their best bandwidth with about one 256-element
+  // tile per workgroup (the resident workgroups then sweep neighbouring tiles together)
+  static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
+  const int ki = mtcocr_handle::sp_index(sp), q = precision;
+  int* caps[4] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki],
+                  &h->cap_upd[q][ki]};
+  int best[4] = {*caps[0], *caps[1], *caps[2], *caps[3]};
+  int best_layout = h->spmv_layout[q][ki];
+  double best_ms[4] = {1e30, 1e30, 1e30, 1e30};
+  rc = with_sp(sp, [&](auto spc) -> int {
+    constexpr int SP = decltype(spc)::value;
+    auto run = [&](auto tag) -> int {
+      using T = decltype(tag);
+      using T2 = typename Vec2<T>::type;
+      {
+        Engine<T, SP> E(h);
+        const long long nv = (long long)E.nb * SP, nm = (long long)std::max(E.m, 1) * SP;
+        const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
+        const int gm = (int)std::min<long long>(2048, (nm + 255) / 256);
+        for (T2* v : {E.s, E.z, E.a, E.p, E.w, E.x, E.x2, E.r, E.ym, E.sm, E.sr[1]})
+          k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+        k_fill<T2><<<gm, 256, 0, st>>>(E.help, nm);
+        E.setup_diagonals(st);
+        HIPC(hipGetLastError());
+      }
+      for (int fam = 0; fam < 4; ++fam) {
+        // SpMV layouts that apply to this batch (the others fall back to one of these)
+        std::vector<int> layouts{0};
+        if (fam == 0 && SP >= 4 && rows_fit(h, SP, int(sizeof(T2)))) {
+          layouts.push_back(1);
+          if (SP <= 16 && blk_lds_bytes<T>(h) > 0) layouts.push_back(2);
+          if (SP == 16 && h->stc_ok) {
+            layouts.push_back(3);
+            layouts.push_back(4);
+          }
+        }
+        int best_kc = h->stc_kc[q][ki];
+        int best_band = h->stl_band[q][ki];
+        for (int layout : layouts) {
+          // the stencil kernel's sweep length: a few lengths, each at every cap
+          std::vector<int> kcs{0};
+          if (layout >= 3) {
+            kcs = {2, 4, 8, 16, h->grid.nz};
+            if (layout == 4) kcs = {8, 16, (h->grid.nz + 1) / 2, h->grid.nz};
+            std::sort(kcs.begin(), kcs.end());
+            kcs.erase(std::unique(kcs.begin(), kcs.end()), kcs.end());
+          }
+          // the LDS-staged kernel's task order: bands of tile columns (0: one band)
+          std::vector<int> bands{h->stl_band[q][ki]};
+          if (layout == 4) bands = {0, 5, 9, 13};
+          for (int kc : kcs) {
+          for (int band : bands) {
+          if (layout >= 3) h->stc_kc[q][ki] = kc;
+          if (layout == 4) h->stl_band[q][ki] = band;
+          // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
+          // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
+          // twice that
+          int per_cu = 0;
+          if (fam == 0 && layout == 3) {
+            if constexpr (SP == 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_stc<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0 && layout == 4) {
+            if constexpr (SP == 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_stl<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0 && layout == 2) {
+            if constexpr (SP >= 4 && SP <= 16)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_blk<T, SP, 0>), kBlock,
+                  blk_lds_bytes<T>(h)));
+          } else if (fam == 0 && layout == 1) {
+            if constexpr (SP >= 4)
+              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, reinterpret_cast<const void*>(k_spmv_rg<T, SP, 0>), kBlock, 0));
+          } else if (fam == 0) {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_spmv<T, SP, 0>), kBlock, 0));
+          } else if (fam == 1) {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_pc_node<T, SP, true>), kBlock, 0));
+          } else if (fam == 2) {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(k_pc_edge_iter<T, SP>), kBlock, 0));
+          } else {
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu,
+                reinterpret_cast<const void*>(k_xr_fused<T, SP, Engine<T, SP>::kLazy>),
+                kBlock, 0));
+          }
+          std::vector<int> cands(std::begin(kCands), std::end(kCands));
+          const int resident = round_up(std::max(1, per_cu * h->n_cu), kXcd);
+          for (int c : {resident, 2 * resident})
+            if (c <= kMaxBlocks) cands.push_back(c);
+          std::sort(cands.begin(), cands.end());
+          cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+          int last_grid = -1;
+          for (int cap : cands) {
+            *caps[fam] = cap;
+            if (fam == 0) h->spmv_layout[q][ki] = layout;
+            Engine<T, SP> E(h);
+            const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : fam == 2 ? E.grid_e
+                                                                                   : E.grid_u;
+            if (grid == last_grid) continue;  // the cap no longer limits this grid
+            last_grid = grid;
+            // each timed unit is the kernel plus the one-block reduction of its partials
+            double* probe_out = h->r64.as<double>();  // scratch: the outer residual buffer
+            auto launch = [&]() {
+              if (fam == 0) {
+                E.template spmv<0>(st, E.s, E.z, (const T2*)nullptr, (T2*)nullptr,
+                                   (T2*)nullptr);
+                int nr;
+                const double* pp = E.template partials<SP * 2>(st, E.grid_s, nr);
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
+              } else if (fam == 1) {
+                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
+                                                                    E.sc, E.nrun);
+              } else if (fam == 2) {
+                // after the node pass (at its tuned shape), as in an iteration: the edge
+                // kernel's gathers of help then find what the node pass left in the caches
+                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
+                                                                    E.sc, E.nrun);
+                k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.w,
+                                                                   E.da, E.help, E.sc, E.nrun,
+                                                                   E.part);
+                int nr;
+                const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
+                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
+              } else {
+                // the pass as the solver runs it (FP64: without x and p, s into the ring)
+                if constexpr (Engine<T, SP>::kLazy)
+                  k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
+                      nullptr, nullptr, nullptr, E.r, E.a, E.s, E.sr[1], E.w, nullptr, E.sm,
+                      E.sc, E.nrun, E.part, E.nb);
+                else
+                  k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(
+                      E.x, E.x2, E.p, E.r, E.a, E.s, E.s, E.w, E.ym_or_null(), E.sm, E.sc,
+                      E.nrun, E.part, E.nb);
+                int nr;
+                const double* pp = E.template partials<SP * 3>(st, E.grid_u, nr);
+                s_reduce_probe<SP * 3><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
+              }
+            };
+            launch();
+            HIPC(hipGetLastError());
+            HIPC(hipEventRecord(h->ev0, st));
+            for (int i = 0; i < reps; ++i) launch();
+            HIPC(hipEventRecord(h->ev1, st));
+            HIPC(hipGetLastError());
+            HIPC(hipEventSynchronize(h->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+            const double avg = double(ms) / reps;
+            if (avg < best_ms[fam]) {
+              best_ms[fam] = avg;
+              best[fam] = cap;
+              if (fam == 0) best_layout = layout;
+              if (fam == 0 && layout >= 3) best_kc = kc;
+              if (fam == 0 && layout == 4) best_band = band;
+            }
+          }
+          }
+          }
+        }
+        *caps[fam] = best[fam];
+        if (fam == 0) {
+          h->spmv_layout[q][ki] = best_layout;
+          h->stc_kc[q][ki] = best_kc;
+          h->stl_band[q][ki] = best_band;
+        }
+      }
+      return 0;
+    };
+    if (precision == MTCOCR_PREC_FP64) return run(double());
+    return run(float());
+  });
+  drop_graph(h);
+  if (rc) return rc;
+  *spmv_ms_out = best_ms[0];
   return 0;
 }
 
@@ -4221,7 +4419,10 @@ int mtcocr_get_launch(mtcocr_handle* h, int precision, int* cap_spmv, int* cap_v
 //   edge streams    k_pc_edge_iter cap (reported as cap_vec), timed after the node pass at
 //                   its tuned cap, as it runs in an iteration
 //   update pass     k_xr_fused, tuned apart (set_launch's cap_vec sets both)
-// Only work buffers are written; the resident right-hand sides and solutions are untouched.
+// An FP64 batch is tuned at its own width and then at every width its compaction narrows it
+// to (sp/2, sp/4, ... 1, on its first systems), so that the narrowed batches also run at tuned
+// shapes. Only work buffers are written; the resident right-hand sides and solutions are
+// untouched.
 int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, int* cap_vec,
                     int* cap_node, int* spmv_layout, double* spmv_ms) {
   if (!h || reps < 1) return MTCOCR_ERR_ARG;
@@ -4231,198 +4432,22 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
   // FP64 batches wider than kGroupWidth iterate in groups of kGroupWidth: tune that width
   const int sp =
       (precision == MTCOCR_PREC_FP64 && h->sp > kGroupWidth) ? kGroupWidth : h->sp;
-  int rc = alloc_work(h, precision, sp);
+  // the batch's own width, then (FP64) every width its compaction narrows it to: sp/2, sp/4, ... 1
+  double ms0 = 0.0;
+  int rc = autotune_width(h, precision, reps, sp, &ms0);
   if (rc) return rc;
-  hipStream_t st = h->stream;
-  std::vector<SysScalars> hs;
-  set_eps_status(h, hs, std::min(h->nsys, sp), sp, 1e-8);
-  for (auto& q : hs) q.mode = MRS_BLEND;  // the update pass's common step
-  HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
-  int one = 1;
-  HIPC(hipMemcpyAsync(h->nrun.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
-  // large grids: the streaming kernels reach their best bandwidth with about one 256-element
-  // tile per workgroup (the resident workgroups then sweep neighbouring tiles together)
-  static const int kCands[] = {512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072};
+  if (precision == MTCOCR_PREC_FP64) {
+    for (int w = sp / 2; w >= 1; w /= 2) {
+      double msw = 0.0;
+      if ((rc = autotune_width(h, precision, reps, w, &msw))) return rc;
+    }
+  }
   const int ki = mtcocr_handle::sp_index(sp), q = precision;
-  int* caps[4] = {&h->cap_spmv[q][ki], &h->cap_node[q][ki], &h->cap_vec[q][ki],
-                  &h->cap_upd[q][ki]};
-  int best[4] = {*caps[0], *caps[1], *caps[2], *caps[3]};
-  int best_layout = h->spmv_layout[q][ki];
-  double best_ms[4] = {1e30, 1e30, 1e30, 1e30};
-  rc = with_sp(sp, [&](auto spc) -> int {
-    constexpr int SP = decltype(spc)::value;
-    auto run = [&](auto tag) -> int {
-      using T = decltype(tag);
-      using T2 = typename Vec2<T>::type;
-      {
-        Engine<T, SP> E(h);
-        const long long nv = (long long)E.nb * SP, nm = (long long)std::max(E.m, 1) * SP;
-        const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
-        const int gm = (int)std::min<long long>(2048, (nm + 255) / 256);
-        for (T2* v : {E.s, E.z, E.a, E.p, E.w, E.x, E.x2, E.r, E.ym, E.sm, E.sr[1]})
-          k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
-        k_fill<T2><<<gm, 256, 0, st>>>(E.help, nm);
-        E.setup_diagonals(st);
-        HIPC(hipGetLastError());
-      }
-      for (int fam = 0; fam < 4; ++fam) {
-        // SpMV layouts that apply to this batch (the others fall back to one of these)
-        std::vector<int> layouts{0};
-        if (fam == 0 && SP >= 4 && rows_fit(h, SP, int(sizeof(T2)))) {
-          layouts.push_back(1);
-          if (SP <= 16 && blk_lds_bytes<T>(h) > 0) layouts.push_back(2);
-          if (SP == 16 && h->stc_ok) {
-            layouts.push_back(3);
-            layouts.push_back(4);
-          }
-        }
-        int best_kc = h->stc_kc[q][ki];
-        int best_band = h->stl_band[q][ki];
-        for (int layout : layouts) {
-          // the stencil kernel's sweep length: a few lengths, each at every cap
-          std::vector<int> kcs{0};
-          if (layout >= 3) {
-            kcs = {2, 4, 8, 16, h->grid.nz};
-            if (layout == 4) kcs = {8, 16, (h->grid.nz + 1) / 2, h->grid.nz};
-            std::sort(kcs.begin(), kcs.end());
-            kcs.erase(std::unique(kcs.begin(), kcs.end()), kcs.end());
-          }
-          // the LDS-staged kernel's task order: bands of tile columns (0: one band)
-          std::vector<int> bands{h->stl_band[q][ki]};
-          if (layout == 4) bands = {0, 5, 9, 13};
-          for (int kc : kcs) {
-          for (int band : bands) {
-          if (layout >= 3) h->stc_kc[q][ki] = kc;
-          if (layout == 4) h->stl_band[q][ki] = band;
-          // candidates: fixed caps, plus the kernel's resident capacity (a grid that runs in
-          // one wave, whose tile walk then sweeps each XCD's rows as one contiguous front) and
-          // twice that
-          int per_cu = 0;
-          if (fam == 0 && layout == 3) {
-            if constexpr (SP == 16)
-              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, reinterpret_cast<const void*>(k_spmv_stc<T, SP, 0>), kBlock, 0));
-          } else if (fam == 0 && layout == 4) {
-            if constexpr (SP == 16)
-              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, reinterpret_cast<const void*>(k_spmv_stl<T, SP, 0>), kBlock, 0));
-          } else if (fam == 0 && layout == 2) {
-            if constexpr (SP >= 4 && SP <= 16)
-              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, reinterpret_cast<const void*>(k_spmv_blk<T, SP, 0>), kBlock,
-                  blk_lds_bytes<T>(h)));
-          } else if (fam == 0 && layout == 1) {
-            if constexpr (SP >= 4)
-              HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, reinterpret_cast<const void*>(k_spmv_rg<T, SP, 0>), kBlock, 0));
-          } else if (fam == 0) {
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(k_spmv<T, SP, 0>), kBlock, 0));
-          } else if (fam == 1) {
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(k_pc_node<T, SP, true>), kBlock, 0));
-          } else if (fam == 2) {
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(k_pc_edge_iter<T, SP>), kBlock, 0));
-          } else {
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu,
-                reinterpret_cast<const void*>(k_xr_fused<T, SP, Engine<T, SP>::kLazy>),
-                kBlock, 0));
-          }
-          std::vector<int> cands(std::begin(kCands), std::end(kCands));
-          const int resident = round_up(std::max(1, per_cu * h->n_cu), kXcd);
-          for (int c : {resident, 2 * resident})
-            if (c <= kMaxBlocks) cands.push_back(c);
-          std::sort(cands.begin(), cands.end());
-          cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
-          int last_grid = -1;
-          for (int cap : cands) {
-            *caps[fam] = cap;
-            if (fam == 0) h->spmv_layout[q][ki] = layout;
-            Engine<T, SP> E(h);
-            const int grid = fam == 0 ? E.grid_s : fam == 1 ? E.grid_n : fam == 2 ? E.grid_e
-                                                                                   : E.grid_u;
-            if (grid == last_grid) continue;  // the cap no longer limits this grid
-            last_grid = grid;
-            // each timed unit is the kernel plus the one-block reduction of its partials
-            double* probe_out = h->r64.as<double>();  // scratch: the outer residual buffer
-            auto launch = [&]() {
-              if (fam == 0) {
-                E.template spmv<0>(st, E.s, E.z, (const T2*)nullptr, (T2*)nullptr,
-                                   (T2*)nullptr);
-                int nr;
-                const double* pp = E.template partials<SP * 2>(st, E.grid_s, nr);
-                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
-              } else if (fam == 1) {
-                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
-                                                                    E.sc, E.nrun);
-              } else if (fam == 2) {
-                // after the node pass (at its tuned shape), as in an iteration: the edge
-                // kernel's gathers of help then find what the node pass left in the caches
-                k_pc_node<T, SP, true><<<E.grid_n, kBlock, 0, st>>>(E.G, E.z, E.a, E.dn, E.help,
-                                                                    E.sc, E.nrun);
-                k_pc_edge_iter<T, SP><<<E.grid_e, kBlock, 0, st>>>(E.G, E.z, E.a, E.w,
-                                                                   E.da, E.help, E.sc, E.nrun,
-                                                                   E.part);
-                int nr;
-                const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
-                s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
-              } else {
-                // the pass as the solver runs it (FP64: without x and p, s into the ring)
-                if constexpr (Engine<T, SP>::kLazy)
-                  k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
-                      nullptr, nullptr, nullptr, E.r, E.a, E.s, E.sr[1], E.w, nullptr, E.sm,
-                      E.sc, E.nrun, E.part, E.nb);
-                else
-                  k_xr_fused<T, SP><<<E.grid_u, kBlock, 0, st>>>(
-                      E.x, E.x2, E.p, E.r, E.a, E.s, E.s, E.w, E.ym_or_null(), E.sm, E.sc,
-                      E.nrun, E.part, E.nb);
-                int nr;
-                const double* pp = E.template partials<SP * 3>(st, E.grid_u, nr);
-                s_reduce_probe<SP * 3><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
-              }
-            };
-            launch();
-            HIPC(hipGetLastError());
-            HIPC(hipEventRecord(h->ev0, st));
-            for (int i = 0; i < reps; ++i) launch();
-            HIPC(hipEventRecord(h->ev1, st));
-            HIPC(hipGetLastError());
-            HIPC(hipEventSynchronize(h->ev1));
-            float ms = 0.f;
-            HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-            const double avg = double(ms) / reps;
-            if (avg < best_ms[fam]) {
-              best_ms[fam] = avg;
-              best[fam] = cap;
-              if (fam == 0) best_layout = layout;
-              if (fam == 0 && layout >= 3) best_kc = kc;
-              if (fam == 0 && layout == 4) best_band = band;
-            }
-          }
-          }
-          }
-        }
-        *caps[fam] = best[fam];
-        if (fam == 0) {
-          h->spmv_layout[q][ki] = best_layout;
-          h->stc_kc[q][ki] = best_kc;
-          h->stl_band[q][ki] = best_band;
-        }
-      }
-      return 0;
-    };
-    if (precision == MTCOCR_PREC_FP64) return run(double());
-    return run(float());
-  });
-  drop_graph(h);
-  if (rc) return rc;
   if (cap_spmv) *cap_spmv = h->cap_spmv[q][ki];
   if (cap_node) *cap_node = h->cap_node[q][ki];
   if (cap_vec) *cap_vec = h->cap_vec[q][ki];
   if (spmv_layout) *spmv_layout = effective_layout(h, q, ki);
-  if (spmv_ms) *spmv_ms = best_ms[0];
+  if (spmv_ms) *spmv_ms = ms0;
   return 0;
 }
 

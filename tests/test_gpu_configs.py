@@ -107,7 +107,11 @@ def test_config3_four_models(O, M):
         xd, sd_ = out["fp64"]
         for s in range(len(cases)):
             rm, rd = true_rel(prs[s], om[s], xm[s]), true_rel(prs[s], om[s], xd[s])
-            assert sm[s]["req"] == 0 and rm < 1e-8, (m.name, s, sm[s], rm)
+            # a system flagged at the residual floor (MTCOCR_FLAG_RES_FLOOR: its FP64 finish met
+            # the stop test at eps/30) is held to the FP64 solver's bound instead
+            floor = sm[s]["flags"] & 2
+            assert sm[s]["req"] == 0 and (rm < 1e-8 or (floor and rm <= FP64_TRUE_BOUND)), \
+                (m.name, s, sm[s], rm)
             assert sd_[s]["req"] == 0 and sd_[s]["residual_rel"] < 1e-8
             assert rd <= FP64_TRUE_BOUND, (m.name, s, sd_[s], rd)
         say(f"  fp64 iterations {[q['iter'] for q in sd_]}; mixed passes "
@@ -188,7 +192,9 @@ def test_config5_large_batch(M):
             r = true_rel(prs[s], om[s], x[s])
             say(f"  {prec} {cases[s][0]:.3g} Hz pol {'XY'[1 - cases[s][1]]}: req {st[s]['req']}"
                 f" it {st[s]['iter']} true rel res {r:.2e} ({st[s]['seconds']:.1f} s batch)")
-            assert st[s]["req"] == 0 and r <= bound
+            # mixed: a system at the residual floor (MTCOCR_FLAG_RES_FLOOR) as FP64
+            b = FP64_TRUE_BOUND if st[s]["flags"] & 2 else bound
+            assert st[s]["req"] == 0 and r <= b
 
 
 DTM1 = Path(__file__).resolve().parent / "golden" / "dtm1_example.json"
