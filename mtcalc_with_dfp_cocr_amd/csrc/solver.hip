@@ -196,6 +196,28 @@ __device__ inline void block_partials(double (&v)[NV], double* partials) {
   }
 }
 
+// block_partials for kernels with 16-lane rows of 8 systems x 2 lanes (lane = 16 g + 2 s + c),
+// once the two lanes of a system hold the same sums: lanes with equal (lane % 16) / 2 share a
+// system. Writes partials[blockIdx.x * (SP*NV) + s*NV + k], SP = 8.
+template <int SP, int NV>
+__device__ inline void block_partials_split(double (&v)[NV], double* partials) {
+  static_assert(SP == 8, "8 systems on 16-lane rows");
+  for (int off = 16; off < 64; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  __shared__ double sh[kBlock / 64][SP * NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < 16 && (lane & 1) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wave][(lane >> 1) * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < SP * NV) {
+    double s = 0.0;
+    for (int wv = 0; wv < kBlock / 64; ++wv) s += sh[wv][threadIdx.x];
+    partials[(size_t)blockIdx.x * (SP * NV) + threadIdx.x] = s;
+  }
+}
+
 // One-block reduction of nblk x P partials into tot[P] (fixed order): G = kScalarThreads / P
 // threads per column each sum a strided subset, then a fixed tree over the G sums (the part
 // of G above its largest power of two folded in first).
@@ -1013,6 +1035,25 @@ __device__ inline void dfma4_bc(double& ar, double& ai, double ca, double cb, do
       : "v"(ca), "v"(cb), "v"(xr), "v"(xi), "v"(nwxi), "v"(wxr), "i"(Ja), "i"(Jb));
 }
 
+// One component per lane (the 8-system stencil kernel, whose 16-lane rows hold 8 systems x
+// (re, im)): acc += c u, and acc += ca u + cb v, with the same broadcasts and FMA order as the
+// component of dfma2_bc / dfma4_bc that the lane stands for.
+template <int J>
+__device__ inline void dfma1_bc(double& acc, double c, double u) {
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(acc)
+      : "v"(c), "v"(u), "i"(J));
+}
+template <int Ja, int Jb>
+__device__ inline void dfma2x_bc(double& acc, double ca, double cb, double u, double v) {
+  asm("s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %1, %3 row_newbcast:%5 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%6 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(acc)
+      : "v"(ca), "v"(cb), "v"(u), "v"(v), "i"(Ja), "i"(Jb));
+}
+
 struct StcView {
   const double* coef;  // nb x kStcCoef, row-major
   int nx, ny, nz;      // nodes per axis
@@ -1332,16 +1373,58 @@ constexpr int kStlMaxVal = stl_imax(stl_nval(0), stl_imax(stl_nval(1), stl_nval(
 static_assert(stl_nval(0) == 79 && stl_nval(1) == 79 && stl_nval(2) == 96,
               "plane sizes of a 4 x 4 tile");
 
+// stl_store for one component (comp 0: re, 1: im) of the (row, system) element at byte offset
+// `off`, as the 8-system stencil kernel's lanes hold them: the same stored values; each lane
+// adds its component's terms of the partial sums (the pair of lanes adds up to stl_store's).
+template <class T, int MODE>
+__device__ inline void stl_store_split(bool ok, unsigned off, int comp, double acc,
+                                       typename Vec2<T>::type xself, __amdgpu_buffer_rsrc_t y_r,
+                                       __amdgpu_buffer_rsrc_t b_r, __amdgpu_buffer_rsrc_t ym_r,
+                                       __amdgpu_buffer_rsrc_t sm_r, double (&v)[2]) {
+  const unsigned o = ok ? off + unsigned(comp) * unsigned(sizeof(T)) : kOob;
+  auto store = [&](__amdgpu_buffer_rsrc_t r, T val) {
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, val), r, int(o), 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, val), r, int(o), 0, 0);
+  };
+  const T xc = ok ? (comp ? xself.y : xself.x) : T(0);  // this component of the row's own x
+  const T xo = ok ? (comp ? xself.x : xself.y) : T(0);  // the other one
+  if (MODE == 0) {
+    const T out = T(acc);
+    store(y_r, out);
+    // (z, x) = (zr xr - zi xi) + i (zr xi + zi xr): comp 0 holds zr, comp 1 holds zi
+    v[0] += comp ? -double(out) * double(xc) : double(out) * double(xc);
+    v[1] += double(out) * double(xo);
+  } else {
+    T bc;
+    if constexpr (sizeof(T) == 8)
+      bc = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(b_r, int(o), 0, 0));
+    else
+      bc = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(b_r, int(o), 0, 0));
+    const T out = T(double(bc) - acc);
+    store(y_r, out);
+    v[0] += double(bc) * double(bc);
+    v[1] += ok ? double(out) * double(out) : 0.0;
+    store(ym_r, xc);
+    store(sm_r, out);
+  }
+}
+
 template <class T, int SP, int MODE, int D>
 __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s, int g, int wv,
-                                 double w, bool run, __amdgpu_buffer_rsrc_t xr,
+                                 int l16, int comp, double w, bool run,
+                                 __amdgpu_buffer_rsrc_t xr,
                                  __amdgpu_buffer_rsrc_t cr, __amdgpu_buffer_rsrc_t yr_,
                                  __amdgpu_buffer_rsrc_t br_, __amdgpu_buffer_rsrc_t ymr,
                                  __amdgpu_buffer_rsrc_t smr, typename Vec2<T>::type* pl,
                                  double (&v)[2]) {
   using T2 = typename Vec2<T>::type;
+  // 16 lanes per row: 16 systems, or (SPLIT) 8 systems x (re, im), each lane one component
+  constexpr bool SPLIT = SP == 8;
   constexpr int NC = kStcCols[D];
-  constexpr int NCR = (kStcCoef + SP - 1) / SP;
+  constexpr int NCR = (kStcCoef + 15) / 16;
   constexpr int LO = D == 2 ? 0 : -1, HI = 1;
   constexpr int PS = kStlMaxVal * SP;              // elements of one plane buffer
   const GridDims G{A.nx, A.ny, A.nz};
@@ -1416,7 +1499,7 @@ __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s
     const long long r = G.base(D) + ((long long)kk * nyd + jr) * nxd + ir;
     const unsigned rb = unsigned(r) * unsigned(kStcCoef * 8);
     static_for<0, NCR>([&](auto Q) {
-      const int slot = Q * SP + s;
+      const int slot = Q * 16 + l16;
       dst[Q] = buf_f64(cr, slot < kStcCoef ? rb + unsigned(slot) * 8u : kOob);
     });
   };
@@ -1468,15 +1551,25 @@ __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s
           constexpr StencilCol c = kStc[D][C];
           const double xr_ = double(xb[B % 2][J].x), xi_ = double(xb[B % 2][J].y);
           const double nwxi = c.dt == D ? -w * xi_ : 0.0, wxr = c.dt == D ? w * xr_ : 0.0;
+          // SPLIT: this lane's component of x and of the i w b term
+          const double u = comp ? xi_ : xr_, vv = comp ? wxr : nwxi;
           static_for<c.lo, c.hi + 1>([&](auto DK) {
             constexpr int dk = decltype(DK)::value;
             constexpr int R = ((P - dk) % 4 + 4) % 4;
             constexpr int o = stc_slot(D, C, dk);
-            if constexpr (c.dt == D)
-              dfma4_bc<o % SP, (o + 1) % SP>(ar[R], ai[R], cf[R][o / SP], cf[R][(o + 1) / SP],
-                                             xr_, xi_, nwxi, wxr);
-            else
-              dfma2_bc<o % SP>(ar[R], ai[R], cf[R][o / SP], xr_, xi_);
+            if constexpr (SPLIT) {
+              if constexpr (c.dt == D)
+                dfma2x_bc<o % 16, (o + 1) % 16>(ar[R], cf[R][o / 16], cf[R][(o + 1) / 16], u,
+                                                vv);
+              else
+                dfma1_bc<o % 16>(ar[R], cf[R][o / 16], u);
+            } else {
+              if constexpr (c.dt == D)
+                dfma4_bc<o % 16, (o + 1) % 16>(ar[R], ai[R], cf[R][o / 16], cf[R][(o + 1) / 16],
+                                               xr_, xi_, nwxi, wxr);
+              else
+                dfma2_bc<o % 16>(ar[R], ai[R], cf[R][o / 16], xr_, xi_);
+            }
           });
         }
       });
@@ -1490,8 +1583,12 @@ __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s
     const bool ok = k >= k0 && k < k1 && row_ok && (run || MODE == 0);
     const long long tt =
         (G.base(D) + ((long long)min(max(k, 0), nzd - 1) * nyd + jr) * nxd + ir) * SP + s;
-    stl_store<T, MODE>(ok, unsigned(tt) * unsigned(sizeof(T2)), ar[R], ai[R], xs[(P + 1) % 2],
-                       yr_, br_, ymr, smr, v);
+    if constexpr (SPLIT)
+      stl_store_split<T, MODE>(ok, unsigned(tt) * unsigned(sizeof(T2)), comp, ar[R],
+                               xs[(P + 1) % 2], yr_, br_, ymr, smr, v);
+    else
+      stl_store<T, MODE>(ok, unsigned(tt) * unsigned(sizeof(T2)), ar[R], ai[R],
+                         xs[(P + 1) % 2], yr_, br_, ymr, smr, v);
     if (more) put(nxt, nxt_buf);
   };
   for (int L = L0; L <= L1; L += 4) {
@@ -1516,13 +1613,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* om
                                                      const SysScalars* __restrict__ sc,
                                                      const int* __restrict__ nrun,
                                                      double* partials) {
-  static_assert(SP == 16, "16-lane rows (the DPP row broadcast)");
+  // 16-lane rows (the DPP row broadcast): 16 systems, or 8 systems x (re, im)
+  static_assert(SP == 16 || SP == 8, "16 or 8 systems");
   using T2 = typename Vec2<T>::type;
   __shared__ T2 pl[2 * kStlMaxVal * SP];
   if (*nrun == 0) return;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-  const int s = lane % SP, g = lane / SP;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int s = SP == 16 ? l16 : l16 >> 1, comp = SP == 16 ? 0 : l16 & 1;
   const bool run = (MODE == 2 ? sc[s].outer_status : sc[s].status) == ST_RUNNING;
   const double w = omega[s];
   double v[2] = {0.0, 0.0};
@@ -1551,16 +1650,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* om
     const int jc = int(rr / wb);
     const int ic = band * A.nbw + int(rr % wb);
     if (d == 0)
-      stl_sweep<T, SP, MODE, 0>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
-                                pl, v);
+      stl_sweep<T, SP, MODE, 0>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+                                ym_r, sm_r, pl, v);
     else if (d == 1)
-      stl_sweep<T, SP, MODE, 1>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
-                                pl, v);
+      stl_sweep<T, SP, MODE, 1>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+                                ym_r, sm_r, pl, v);
     else
-      stl_sweep<T, SP, MODE, 2>(A, ic, jc, kc, s, g, wv, w, run, xr, cr, y_r, b_r, ym_r, sm_r,
-                                pl, v);
+      stl_sweep<T, SP, MODE, 2>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+                                ym_r, sm_r, pl, v);
   }
-  block_partials<SP, 2>(v, partials);
+  if constexpr (SP == 8) {
+    v[0] += __shfl_xor(v[0], 1, 64);  // the (re, im) lane pair of a system
+    v[1] += __shfl_xor(v[1], 1, 64);
+    block_partials_split<SP, 2>(v, partials);
+  } else {
+    block_partials<SP, 2>(v, partials);
+  }
 }
 
 // The folded preconditioner's arithmetic, shared by the node and edge kernels and the fused
@@ -2898,6 +3003,14 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  double* part) {
   if (!rows_fit(h, SP, int(sizeof(typename Vec2<TV>::type)))) layout = 0;
   if constexpr (SP >= 4 && SP <= 16) {
+    if constexpr (SP == 8) {
+      if (layout == 4 && h->stc_ok) {
+        k_spmv_stl<TV, SP, MODE><<<grid, kBlock, 0, st>>>(
+            stl_view(h, stc_kc_of<TV, SP>(h), stl_band_of<TV, SP>(h)), omega, x, y, b, ym, sm,
+            sc, nrun, part);
+        return;
+      }
+    }
     if constexpr (SP == 16) {
       if (layout == 3 && h->stc_ok) {
         k_spmv_stc<TV, SP, MODE><<<grid, kBlock, 0, st>>>(stc_view<SP>(h, stc_kc_of<TV, SP>(h)),
@@ -2911,7 +3024,7 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
         return;
       }
     }
-    if (layout >= 3) layout = 2;  // the stencil layouts serve 16-lane rows only
+    if (layout >= 3) layout = 2;  // the stencil layouts serve 16 (and, LDS-staged, 8) systems
     const int lds = blk_lds_bytes<TV>(h);
     if (layout == 2 && lds > 0) {
       k_spmv_blk<TV, SP, MODE><<<grid, kBlock, lds, st>>>(blk_view(h), h->halo_max, omega, x, y,
@@ -3763,6 +3876,7 @@ int autotune_width(mtcocr_handle* h, int precision, int reps, int sp, double* sp
             layouts.push_back(3);
             layouts.push_back(4);
           }
+          if (SP == 8 && h->stc_ok) layouts.push_back(4);
         }
         int best_kc = h->stc_kc[q][ki];
         int best_band = h->stl_band[q][ki];
@@ -3791,7 +3905,7 @@ int autotune_width(mtcocr_handle* h, int precision, int reps, int sp, double* sp
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_stc<T, SP, 0>), kBlock, 0));
           } else if (fam == 0 && layout == 4) {
-            if constexpr (SP == 16)
+            if constexpr (SP == 16 || SP == 8)
               HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &per_cu, reinterpret_cast<const void*>(k_spmv_stl<T, SP, 0>), kBlock, 0));
           } else if (fam == 0 && layout == 2) {
@@ -4356,7 +4470,7 @@ namespace {
 // the row pack where they do not apply)
 int effective_layout(const mtcocr_handle* h, int q, int k) {
   int l = h->spmv_layout[q][k];
-  if (l >= 3 && !(h->stc_ok && k == 4)) l = 2;
+  if (l >= 3 && !(h->stc_ok && (k == 4 || (k == 3 && l == 4)))) l = 2;
   if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
   if (!rows_fit(h, 1 << k, q == 0 ? 16 : 8)) l = 0;
   return l;

@@ -92,14 +92,55 @@ def test_spmv_parity_batch_widths(O, M, c1_system, c1_mesh, nsys, layout):
                 assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nsys", [5, 8])
+def test_spmv_stencil_lds_8_systems(M, c1_model, c1_mesh, nsys):
+    """The LDS-staged stencil SpMV at batch width 8 (5 systems padded to 8, or 8): its 16-lane
+    rows hold 8 systems x (re, im), each lane one component, with the same FMAs in the same
+    order as the 16-system kernel's, so system 0's product equals that of a 16-system batch
+    bit for bit, FP64 and FP32 vectors, at a sweep of 3 levels and of whole columns; and
+    matches an exact K + i omega S product."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    say(f"stencil_lds SpMV at width 8, {nsys} systems")
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    x = np.random.default_rng(23).standard_normal(2 * p.nb)
+    import scipy.sparse as sp
+
+    rows = np.repeat(np.arange(p.nb), np.diff(p.ig))
+    full = lambda d, g: (sp.csr_matrix((g, (rows, p.jg)), shape=(p.nb, p.nb))
+                         + sp.csr_matrix((g, (p.jg, rows)), shape=(p.nb, p.nb)) + sp.diags(d))
+    ref = M.to_interleaved((full(kd, kg) + 1j * om[0] * full(sd, sg)) @ M.to_complex(x))
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, ORDER_GRID)
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        B.set_launch(spmv_layout="stencil_lds")
+        for kc in (3, len(c1_model.z)):
+            B.set_stencil_sweep(kc)
+            for prec, tol in (("fp64", 1e-13), ("mixed", 2e-6)):
+                B.set_rhs(prs, om)
+                y16 = B.apply("spmv", x, prec)
+                B.set_rhs(prs[:nsys], om[:nsys])
+                assert B.get_launch(prec)["spmv_layout"] == "stencil_lds"
+                y8 = B.apply("spmv", x, prec)
+                assert np.abs(y8 - ref).max() <= tol * np.abs(ref).max(), (kc, prec)
+                assert np.array_equal(y8, y16), (kc, prec)
+
+
 @pytest.mark.parametrize("layout", ["stencil", "stencil_lds"])
 @pytest.mark.parametrize("nsys", [9, 16])
 def test_spmv_stencil_layout(O, M, c1_model, c1_mesh, nsys, layout):
     """The structured stencil SpMVs (matrix and vectors in the grid order of the box mesh; wave
     tasks, or 4 x 4 tiles with x staged in LDS): system 0 of a batch of 9 or 16 (both run
     16-lane rows) against an exact K + i omega S product, FP64 and FP32 vectors, at a sweep of
-    3 levels (chunks that do not divide the mesh) and of whole columns; narrower batches fall
-    back to the row blocks or the row pack."""
+    3 levels (chunks that do not divide the mesh) and of whole columns; at 8 systems the
+    LDS-staged layout stays, the wave-task one falls back to the row blocks or the row pack."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
@@ -135,10 +176,12 @@ def test_spmv_stencil_layout(O, M, c1_model, c1_mesh, nsys, layout):
                     B.set_launch(spmv_layout="stencil")
                     assert np.array_equal(B.apply("spmv", x, prec), got), (kc, prec)
                     B.set_launch(spmv_layout=layout)
-        # 8 systems: the row blocks where their halos fit (not in the grid order here), else
-        # the row pack
+        # 8 systems: the LDS-staged stencil serves them too (8 systems x (re, im) per 16-lane
+        # row); the wave-task stencil does not, and falls back to the row blocks where their
+        # halos fit (not in the grid order here), else the row pack
         B.set_rhs(prs[:8], om[:8])
-        assert B.get_launch("fp64")["spmv_layout"] in ("rowblock", "rowpack")
+        expect = ("stencil_lds",) if layout == "stencil_lds" else ("rowblock", "rowpack")
+        assert B.get_launch("fp64")["spmv_layout"] in expect
 
 
 def test_stencil_batched_solve(O, M, c1_model, c1_mesh):
