@@ -170,10 +170,6 @@ def run_config(args):
     else:
         m = M.ellipsoid_model(256, 256, 128, seed=args.seed)
         freqs = M.frequencies(1e-2, 1e1, 4)
-        if args.order_block == -1 and not args.order_given:
-            # 8 systems: the pencil order's row-block SpMV (41 s per batch) beats the grid
-            # order's, whose 8-system SpMV is the row pack (46 s)
-            args.order_block = 16
         runs.append(batched_only(m, freqs, args, m.name))
         workload = ("256x256x128 cells, layered background + 10 ellipsoids, 4 freqs 1e-2..1e1 Hz"
                     " x 2 pol = 8 systems, one batch")
@@ -343,8 +339,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--order-block", type=int, default=-1,
                     help="device row order: -1 the grid order of the box mesh (enables the "
-                         "stencil SpMV), b > 0 pencils of b x b nodes, 0 the file order; "
-                         "config 5 (8 systems) uses 16 unless given")
+                         "stencil SpMV), b > 0 pencils of b x b nodes, 0 the file order")
     ap.add_argument("--tune-reps", type=int, default=10,
                     help="launches per candidate shape in the autotuner")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
@@ -362,7 +357,6 @@ def main():
                          "x/p pass (--spmv-reps launches each) and the 1 GiB copy calibration "
                          "kernel; prints one line")
     args = ap.parse_args()
-    args.order_given = any(a.startswith("--order-block") for a in sys.argv[1:])
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

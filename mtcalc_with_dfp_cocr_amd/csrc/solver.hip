@@ -1415,8 +1415,7 @@ __device__ inline void stl_store_split(bool ok, unsigned off, int comp, double a
 template <class T, int SP, int MODE, int D>
 __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s, int g, int wv,
                                  int l16, int comp, double w, bool run,
-                                 __amdgpu_buffer_rsrc_t xr,
-                                 __amdgpu_buffer_rsrc_t cr, __amdgpu_buffer_rsrc_t yr_,
+                                 __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t yr_,
                                  __amdgpu_buffer_rsrc_t br_, __amdgpu_buffer_rsrc_t ymr,
                                  __amdgpu_buffer_rsrc_t smr, typename Vec2<T>::type* pl,
                                  double (&v)[2]) {
@@ -1494,13 +1493,17 @@ __device__ inline void stl_sweep(const StcView& A, int ic, int jc, int kc, int s
                        (c.di - stl_rng(D, c.dt, 0))) * SP;
     return buf[pw[c.dt] + k];
   };
+  // the coefficients of direction D's box through a resource of their own, addressed from
+  // the box's first row: 32-bit offsets then reach tables beyond 4 GiB (each box under it)
+  const __amdgpu_buffer_rsrc_t cb = buf_rsrc(A.coef + G.base(D) * kStcCoef,
+                                             G.count(D) * kStcCoef * 8);
   auto load_coef = [&](double (&dst)[NCR], int k) {
     const int kk = min(max(k, 0), nzd - 1);
-    const long long r = G.base(D) + ((long long)kk * nyd + jr) * nxd + ir;
+    const long long r = ((long long)kk * nyd + jr) * nxd + ir;  // row within the box
     const unsigned rb = unsigned(r) * unsigned(kStcCoef * 8);
     static_for<0, NCR>([&](auto Q) {
       const int slot = Q * 16 + l16;
-      dst[Q] = buf_f64(cr, slot < kStcCoef ? rb + unsigned(slot) * 8u : kOob);
+      dst[Q] = buf_f64(cb, slot < kStcCoef ? rb + unsigned(slot) * 8u : kOob);
     });
   };
   constexpr int NBC = 4;
@@ -1627,7 +1630,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* om
   double v[2] = {0.0, 0.0};
   const long long vbytes = (long long)A.nb * SP * (long long)sizeof(T2);
   const __amdgpu_buffer_rsrc_t xr = buf_rsrc(x, vbytes);
-  const __amdgpu_buffer_rsrc_t cr = buf_rsrc(A.coef, (long long)A.nb * kStcCoef * 8);
   const __amdgpu_buffer_rsrc_t y_r = buf_rsrc(y, vbytes);
   const __amdgpu_buffer_rsrc_t b_r = buf_rsrc(b, b ? vbytes : 0);
   const __amdgpu_buffer_rsrc_t ym_r = buf_rsrc(ym, ym ? vbytes : 0);
@@ -1650,13 +1652,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stl(StcView A, const double* om
     const int jc = int(rr / wb);
     const int ic = band * A.nbw + int(rr % wb);
     if (d == 0)
-      stl_sweep<T, SP, MODE, 0>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+      stl_sweep<T, SP, MODE, 0>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, y_r, b_r,
                                 ym_r, sm_r, pl, v);
     else if (d == 1)
-      stl_sweep<T, SP, MODE, 1>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+      stl_sweep<T, SP, MODE, 1>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, y_r, b_r,
                                 ym_r, sm_r, pl, v);
     else
-      stl_sweep<T, SP, MODE, 2>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, cr, y_r, b_r,
+      stl_sweep<T, SP, MODE, 2>(A, ic, jc, kc, s, g, wv, l16, comp, w, run, xr, y_r, b_r,
                                 ym_r, sm_r, pl, v);
   }
   if constexpr (SP == 8) {
@@ -2729,6 +2731,9 @@ struct mtcocr_handle {
   int stl_band[2][7] = {};
   DevBuf stc;
   bool stc_ok = false;
+  // the table is above 4 GiB (each direction's box under it): only the LDS-staged kernel, which
+  // addresses it box by box, reads it
+  bool stc_big = false;
 
   // matrix (shared index arrays, FP64 and FP32 values)
   bool have_matrix = false;
@@ -3002,6 +3007,7 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
                  typename Vec2<TV>::type* sm, const SysScalars* sc, const int* nrun,
                  double* part) {
   if (!rows_fit(h, SP, int(sizeof(typename Vec2<TV>::type)))) layout = 0;
+  if (layout == 3 && h->stc_big) layout = 4;  // the wave-task kernel cannot address the table
   if constexpr (SP >= 4 && SP <= 16) {
     if constexpr (SP == 8) {
       if (layout == 4 && h->stc_ok) {
@@ -3012,7 +3018,7 @@ void launch_spmv(mtcocr_handle* h, int layout, int grid, hipStream_t st, const d
       }
     }
     if constexpr (SP == 16) {
-      if (layout == 3 && h->stc_ok) {
+      if (layout == 3 && h->stc_ok && !h->stc_big) {
         k_spmv_stc<TV, SP, MODE><<<grid, kBlock, 0, st>>>(stc_view<SP>(h, stc_kc_of<TV, SP>(h)),
                                                           omega, x, y, b, ym, sm, sc, nrun, part);
         return;
@@ -3811,12 +3817,15 @@ int upload_matrix(mtcocr_handle* h, const SlicedEll& E, bool ks) {
   h->nblk = E.hoff.empty() ? 0 : int(E.hoff.size()) - 1;
   h->stc_ok = false;
   h->stc.release();
+  h->stc_big = false;
   if (h->grid_order) {
     std::vector<double> coef;
-    if (stencil_from_rows(E, h->grid, coef) == 0 &&
-        (long long)E.nb * kStcCoef * 8 <= (long long)kOob) {
+    long long box = 0;  // the largest direction box's coefficient bytes
+    for (int d = 0; d < 3; ++d) box = std::max<long long>(box, h->grid.count(d) * kStcCoef * 8);
+    if (stencil_from_rows(E, h->grid, coef) == 0 && box <= (long long)kOob) {
       if ((rc = upload(h, h->stc, coef, st))) return rc;
       h->stc_ok = true;
+      h->stc_big = (long long)E.nb * kStcCoef * 8 > (long long)kOob;
     }
   }
   HIPC(hipStreamSynchronize(st));
@@ -3873,7 +3882,7 @@ int autotune_width(mtcocr_handle* h, int precision, int reps, int sp, double* sp
           layouts.push_back(1);
           if (SP <= 16 && blk_lds_bytes<T>(h) > 0) layouts.push_back(2);
           if (SP == 16 && h->stc_ok) {
-            layouts.push_back(3);
+            if (!h->stc_big) layouts.push_back(3);
             layouts.push_back(4);
           }
           if (SP == 8 && h->stc_ok) layouts.push_back(4);
@@ -4470,6 +4479,7 @@ namespace {
 // the row pack where they do not apply)
 int effective_layout(const mtcocr_handle* h, int q, int k) {
   int l = h->spmv_layout[q][k];
+  if (l == 3 && h->stc_big) l = 4;  // the wave-task kernel cannot address the table
   if (l >= 3 && !(h->stc_ok && (k == 4 || (k == 3 && l == 4)))) l = 2;
   if (l == 2 && (q == 0 ? blk_lds_bytes<double>(h) : blk_lds_bytes<float>(h)) == 0) l = 1;
   if (!rows_fit(h, 1 << k, q == 0 ? 16 : 8)) l = 0;
