@@ -1914,7 +1914,14 @@ struct PgView {
   int nb;
 };
 
-template <class T, int SP>
+//
+// HELP_OUT (the FP64 solver): instead of w, the kernel writes help of the nodes it owns (interior
+// nodes of its tile at its owned levels) to `w`, as a node vector [interior node][system]; the
+// update pass forms w = D_A^-1 a + G^T help of each edge from it (k_xr_fused<.., .., true,
+// true>), with the same arithmetic, so the 16-byte w per edge and system is neither written here
+// nor read there (help is 16 bytes per node and system, about a third of an edge's). The partial
+// (a, w) is formed here as before.
+template <class T, int SP, bool HELP_OUT = false>
 __global__ __launch_bounds__(kBlock) void k_pc_grid(
     PgView P, const typename Vec2<T>::type* __restrict__ z,
     const typename Vec2<T>::type* __restrict__ ain, typename Vec2<T>::type* __restrict__ aout,
@@ -1932,8 +1939,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
   const double br = run ? sc[s].be_re : 0.0, bi = run ? sc[s].be_im : 0.0;
   const long long vb = (long long)P.nb * SP * (long long)sizeof(T2);
   const __amdgpu_buffer_rsrc_t zr = buf_rsrc(z, vb), ar = buf_rsrc(ain, vb);
-  const __amdgpu_buffer_rsrc_t aw = buf_rsrc(aout, vb), wr = buf_rsrc(w, vb);
   const long long m = (long long)(P.nx - 2) * (P.ny - 2) * (P.nz - 2);
+  const __amdgpu_buffer_rsrc_t aw = buf_rsrc(aout, vb);
+  const __amdgpu_buffer_rsrc_t wr = buf_rsrc(w, HELP_OUT ? m * SP * (long long)sizeof(T2) : vb);
   const __amdgpu_buffer_rsrc_t dr = buf_rsrc(dn, m * SP * (long long)sizeof(T2));
   const GridDims G{P.nx, P.ny, P.nz};
   auto store = [&](__amdgpu_buffer_rsrc_t r, unsigned off, T2 val) {
@@ -2069,7 +2077,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
           o.x = yr;
           o.y = yi;
           store(aw, off, an);
-          store(wr, off, o);
+          if (!HELP_OUT) store(wr, off, o);
           if (off != kOob) {
             v[0] += double(an.x) * double(yr) - double(an.y) * double(yi);
             v[1] += double(an.x) * double(yi) + double(an.y) * double(yr);
@@ -2084,6 +2092,13 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
         if (L > k0)
           finish(eoff(2, i, j, L - 1), zo[r], dinv(2, i, j, L - 1), interior(i, j, L - 1), prv[c],
                  own_n, hn);
+        if constexpr (HELP_OUT) {
+          // help of the node itself at an owned level (the level above the chunk is the next
+          // chunk's)
+          const long long mi =
+              (i - 1) + (long long)(P.nx - 2) * ((j - 1) + (long long)(P.ny - 2) * (L - 1));
+          store(wr, own_n && L < k1 ? unsigned((mi * SP + s) * (long long)sizeof(T2)) : kOob, hn);
+        }
         zo[r] = zn[r];
       }
     }
@@ -2109,15 +2124,28 @@ __global__ __launch_bounds__(kBlock) void k_pc_grid(
 // (s_src -> s_vec) instead of overwriting s_{k-1}, which the x/p recurrence still needs. The
 // pass then moves 8 vectors instead of 12. Eager mode passes s_src == s_vec (each element is
 // read before it is written, by the same thread).
-template <class T, int SP, bool LAZY = false>
+//
+// RECW (with LAZY, after k_pc_grid<.., .., true>): w is not read but formed here, as
+// w = D_A^-1 a + G^T help (FoldedPreconditioner.cpp:616-629) from this edge's a, its diagonal
+// and the help of its two end nodes, which `w` then points to (a node vector
+// [interior node][system], `wrec` its box): the arithmetic of the fused preconditioner's edge
+// step in the same order, so the same bits.
+struct WRec {
+  int nx = 0, ny = 0, nz = 0;  // nodes per axis of the box (the matrix in the grid order)
+  DiagA da{};
+};
+
+template <class T, int SP, bool LAZY = false, bool RECW = false>
 __global__ __launch_bounds__(kBlock) void k_xr_fused(
     const typename Vec2<T>::type* __restrict__ xin, typename Vec2<T>::type* __restrict__ xout,
     typename Vec2<T>::type* __restrict__ p, typename Vec2<T>::type* __restrict__ r,
     const typename Vec2<T>::type* __restrict__ a, const typename Vec2<T>::type* s_src,
     typename Vec2<T>::type* s_vec, const typename Vec2<T>::type* __restrict__ w,
     typename Vec2<T>::type* __restrict__ ym, typename Vec2<T>::type* __restrict__ sm,
-    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, double* partials, int nb) {
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, double* partials, int nb,
+    WRec wrec = WRec{}) {
   using T2 = typename Vec2<T>::type;
+  static_assert(!RECW || LAZY, "w is formed in the pass of the FP64 solver only");
   if (*nrun == 0) return;
   const long long N = (long long)nb * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
@@ -2146,6 +2174,20 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
     // this form keeps more bytes in flight per wave.
     struct Elem {
       T2 x, r, sm, s, p, a, w;
+      T2 hlo, hup, dinv;  // RECW: help of the edge's end nodes, D_A^-1
+      bool lo_in, up_in;
+    };
+    // RECW: the edge's box (direction d) and position, its end nodes and their help
+    const GridDims G{wrec.nx, wrec.ny, wrec.nz};
+    const long long mnodes = (long long)(wrec.nx - 2) * (wrec.ny - 2) * (wrec.nz - 2);
+    const __amdgpu_buffer_rsrc_t hr =
+        buf_rsrc(RECW ? w : nullptr, RECW ? mnodes * SP * (long long)sizeof(T2) : 0);
+    auto interior = [&](int i, int j, int k) {
+      return i >= 1 && j >= 1 && k >= 1 && i <= G.nx - 2 && j <= G.ny - 2 && k <= G.nz - 2;
+    };
+    auto hoff = [&](bool in, int i, int j, int k) -> unsigned {
+      const long long mi = (i - 1) + (long long)(G.nx - 2) * ((j - 1) + (long long)(G.ny - 2) * (k - 1));
+      return in ? unsigned((mi * SP + s) * (long long)sizeof(T2)) : kOob;
     };
     auto load = [&](long long t, Elem& E) {
       if (!LAZY) E.x = ld_nt(xsrc + t);
@@ -2155,10 +2197,38 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
         E.s = ld_nt(s_src + t);
         if (!LAZY) E.p = ld_nt(p + t);
         E.a = ld_nt(a + t);
-        E.w = ld_nt(w + t);
+        if constexpr (RECW) {
+          const long long row = t / SP;
+          const long long c0 = G.count(0), c1 = G.count(1);
+          const int d = row < c0 ? 0 : row < c0 + c1 ? 1 : 2;
+          const long long rr = row - G.base(d);
+          const int i = int(rr % G.nxd(d));
+          const long long q = rr / G.nxd(d);
+          const int j = int(q % G.nyd(d)), k = int(q / G.nyd(d));
+          const int iu = i + (d == 0), ju = j + (d == 1), ku = k + (d == 2);
+          E.lo_in = interior(i, j, k);
+          E.up_in = interior(iu, ju, ku);
+          E.hlo = buffer_gather<T2>(hr, hoff(E.lo_in, i, j, k));
+          E.hup = buffer_gather<T2>(hr, hoff(E.up_in, iu, ju, ku));
+          E.dinv = diag_inv<T>(wrec.da, int(row), s);
+        } else {
+          E.w = ld_nt(w + t);
+        }
       }
     };
     auto update = [&](long long t, Elem& E) {
+      if constexpr (RECW) {
+        if (step2) {
+          // as k_pc_grid's edge step: w = a D_A^-1, - help(lower)/2, + help(upper)/2
+          const T half = T(0.5);
+          T yr, yi;
+          pc_cmul(E.a.x, E.a.y, E.dinv, yr, yi);
+          if (E.lo_in) pc_acc(yr, yi, -half, E.hlo);
+          if (E.up_in) pc_acc(yr, yi, half, E.hup);
+          E.w.x = yr;
+          E.w.y = yi;
+        }
+      }
       T2 smv = E.sm;
       if (mode == MRS_BLEND) {
         if (!LAZY && ym) {
@@ -3165,6 +3235,16 @@ struct Engine {
     }
   }
 
+  // what the update pass needs to form w after the fused preconditioner (its box, D_A)
+  WRec wrec() const {
+    WRec q;
+    q.nx = pg.nx;
+    q.ny = pg.ny;
+    q.nz = pg.nz;
+    q.da = da;
+    return q;
+  }
+
   // The MRS-smoothed iterate: the FP32 inner solve of the mixed solver returns it as its
   // correction; the FP64 solver returns x unsmoothed, as the reference does (its MRS keeps y
   // but nothing reads it back), so there it is not kept (null: no traffic).
@@ -3227,8 +3307,12 @@ struct Engine {
       if (use_pg) {
         // a alternates between the two buffers: odd iterations read a and write a2
         acur = odd ? a2 : a;
-        k_pc_grid<T, SP><<<grid_pg, kBlock, 0, st>>>(pg, z, odd ? a : a2, acur, w, dn, da, sc,
-                                                     nrun, part);
+        if constexpr (kLazy)  // help to the node vector; the update pass forms w from it
+          k_pc_grid<T, SP, true><<<grid_pg, kBlock, 0, st>>>(pg, z, odd ? a : a2, acur, help, dn,
+                                                             da, sc, nrun, part);
+        else
+          k_pc_grid<T, SP><<<grid_pg, kBlock, 0, st>>>(pg, z, odd ? a : a2, acur, w, dn, da, sc,
+                                                       nrun, part);
         pp = partials<SP * 2>(st, grid_pg, nr);
       }
     }
@@ -3241,9 +3325,19 @@ struct Engine {
     T2* s_next = s;  // s_k
     if constexpr (kLazy) {
       s_next = sr[(c + 1) & 7];
-      k_xr_fused<T, SP, true><<<grid_u, kBlock, 0, st>>>(nullptr, nullptr, nullptr, r, acur,
-                                                         sr[c & 7], s_next, w, nullptr, sm, sc,
-                                                         nrun, part, nb);
+      bool formed = false;  // w formed from help in the pass (after k_pc_grid)
+      if constexpr (pg_width(SP)) {
+        if (use_pg) {
+          k_xr_fused<T, SP, true, true><<<grid_u, kBlock, 0, st>>>(
+              nullptr, nullptr, nullptr, r, acur, sr[c & 7], s_next, help, nullptr, sm, sc, nrun,
+              part, nb, wrec());
+          formed = true;
+        }
+      }
+      if (!formed)
+        k_xr_fused<T, SP, true><<<grid_u, kBlock, 0, st>>>(nullptr, nullptr, nullptr, r, acur,
+                                                           sr[c & 7], s_next, w, nullptr, sm, sc,
+                                                           nrun, part, nb);
     } else {
       k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, acur, s, s,
                                                    w, ym_or_null(), sm, sc, nrun, part, nb);
