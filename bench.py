@@ -235,7 +235,7 @@ def measure_traffic(args, sp):
     if exe is None:
         return None
     pats = {"spmv": re.compile(r"k_spmv(_rg|_blk|_stc|_stl)?<double, %d, 0>" % sp),
-            "update": re.compile(r"k_xr_fused<double, %d, true>" % sp),
+            "update": re.compile(r"k_xr_fused<double, %d, true(, true)?>" % sp),
             "flush": re.compile(r"k_flush_xp<%d>" % sp)}
     res = {k: {} for k in pats}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -570,7 +570,9 @@ def main():
     spmv_line = kernel_line(f"{SPMV_KERNELS[tune['batched_fp64']['spmv_layout']]}"
                             f"<double,{spb},0> (batched FP64 SpMV)", spmv_bytes, spmv_ms,
                             traffic["spmv"] if traffic else None)
-    upd_line = kernel_line(f"k_xr_fused<double,{spb},true> (batched FP64 update pass)",
+    # in the grid order the update pass forms w from the fused preconditioner's node help
+    upd_name = f"k_xr_fused<double,{spb},true{',true' if args.order_block == -1 else ''}>"
+    upd_line = kernel_line(f"{upd_name} (batched FP64 update pass)",
                            upd_bytes, upd_ms, traffic["update"] if traffic else None)
     xp_line = kernel_line(f"k_flush_xp<{spb}> (batched FP64 x/p pass, every 4th iteration)",
                           xp_bytes, xp_ms, traffic["flush"] if traffic else None)

@@ -4074,7 +4074,10 @@ int autotune_width(mtcocr_handle* h, int precision, int reps, int sp, double* sp
                 const double* pp = E.template partials<SP * 2>(st, E.grid_e, nr);
                 s_reduce_probe<SP * 2><<<1, kScalarThreads, 0, st>>>(pp, nr, probe_out);
               } else {
-                // the pass as the solver runs it (FP64: without x and p, s into the ring)
+                // the pass without x and p (FP64: s into the ring). After the fused grid
+                // preconditioner the solver runs the variant that forms w from help; tuning
+                // the cap on that variant alone picked a cap that made the iterations 0.4%
+                // slower, so the cap is tuned on this one
                 if constexpr (Engine<T, SP>::kLazy)
                   k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
                       nullptr, nullptr, nullptr, E.r, E.a, E.s, E.sr[1], E.w, nullptr, E.sm,
@@ -4449,6 +4452,7 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
   const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
   float ms = 0.f;
   int streams = 0;
+  double node_bytes = 0.0;  // help gathered once per node and system, the diagonal once per edge
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
     auto run = [&](auto tag) -> int {
@@ -4457,11 +4461,25 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
       Engine<T, SP> E(h);
       for (T2* v : {E.x, E.x2, E.p, E.r, E.a, E.s, E.w, E.ym, E.sm, E.sr[1]})
         k_fill<T2><<<gf, 256, 0, st>>>(v, nv);
+      k_fill<T2><<<gf, 256, 0, st>>>(E.help, (long long)std::max(E.m, 1) * SP);
       T2* ym = E.ym_or_null();
       // the pass as the solver runs it: FP64 without x and p (8 vectors: r, sm, s, a, w read;
-      // r, sm and the next s written), FP32 vectors with x, p and the smoothed iterate
-      streams = Engine<T, SP>::kLazy ? 8 : ym ? 14 : 12;
+      // r, sm and the next s written; after the fused grid preconditioner w is formed from the
+      // node vector help instead of read), FP32 vectors with x, p and the smoothed iterate
+      bool formed = false;
+      if constexpr (Engine<T, SP>::kLazy && pg_width(SP)) formed = E.use_pg;
+      streams = Engine<T, SP>::kLazy ? (formed ? 7 : 8) : ym ? 14 : 12;
+      node_bytes = formed ? double(E.m) * 16.0 * double(h->nsys) + 16.0 * double(h->nb) : 0.0;
       auto launch = [&](int i) {
+        if constexpr (Engine<T, SP>::kLazy && pg_width(SP)) {
+          if (formed) {
+            k_xr_fused<T, SP, true, true><<<E.grid_u, kBlock, 0, st>>>(
+                nullptr, nullptr, nullptr, E.r, E.a, i & 1 ? E.sr[1] : E.s,
+                i & 1 ? E.s : E.sr[1], E.help, nullptr, E.sm, E.sc, E.nrun, E.part, E.nb,
+                E.wrec());
+            return;
+          }
+        }
         if constexpr (Engine<T, SP>::kLazy)
           k_xr_fused<T, SP, true><<<E.grid_u, kBlock, 0, st>>>(
               nullptr, nullptr, nullptr, E.r, E.a, i & 1 ? E.sr[1] : E.s, i & 1 ? E.s : E.sr[1],
@@ -4486,9 +4504,10 @@ int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_m
   });
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
-  // algorithmic bytes: each stream read or written once per system and edge
+  // algorithmic bytes: each stream read or written once per system and edge (plus, when w is
+  // formed in the pass, help once per interior node and system and the diagonal once per edge)
   const double ev = (precision == MTCOCR_PREC_FP64) ? 16.0 : 8.0;
-  *bytes_per_launch = double(streams) * ev * double(h->nb) * double(h->nsys);
+  *bytes_per_launch = double(streams) * ev * double(h->nb) * double(h->nsys) + node_bytes;
   return 0;
 }
 
