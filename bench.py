@@ -15,7 +15,7 @@ synchronisations). Reported beside it:
                  systems whose refinement stalls
 The roofline is that of the dominant kernel per iteration (the longer of the batched FP64
 update pass and the batched FP64 SpMV), timed with HIP events on the solver's stream; the SpMV,
-the update pass and the x/p pass (every fourth iteration) are each reported with their HBM
+the update pass and the x/p pass (every eighth iteration) are each reported with their HBM
 traffic from rocprofv3 counters. The CPU baseline is the oracle (FP64 CPU restatement of the
 reference's COCR_FP) run to the stop test on one system of the same workload, on the host's
 cores.
@@ -574,7 +574,7 @@ def main():
     upd_name = f"k_xr_fused<double,{spb},true{',true' if args.order_block == -1 else ''}>"
     upd_line = kernel_line(f"{upd_name} (batched FP64 update pass)",
                            upd_bytes, upd_ms, traffic["update"] if traffic else None)
-    xp_line = kernel_line(f"k_flush_xp<{spb}> (batched FP64 x/p pass, every 4th iteration)",
+    xp_line = kernel_line(f"k_flush_xp<{spb}> (batched FP64 x/p pass, every 8th iteration)",
                           xp_bytes, xp_ms, traffic["flush"] if traffic else None)
     # the dominant kernel: both run once per iteration, the longer one
     roof = upd_line if upd_ms > spmv_ms else spmv_line

@@ -255,16 +255,16 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
  * in the FP32-vector inner solve of the mixed solver also p = s + beta p, x += alpha p and the
  * smoothed x) over the systems of the current batch, `reps` launches on filled work vectors,
  * and the algorithmic bytes per launch (each stream read or written once per system and edge:
- * 8 streams in FP64, whose x and p are updated by k_flush_xp every fourth iteration; 14 with
+ * 8 streams in FP64, whose x and p are updated by k_flush_xp every eighth iteration; 14 with
  * the FP32 vectors). */
 int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_ms,
                         double* bytes_per_launch);
 
 /* Kernel-level timing hook, no counterpart in the reference (pcocr_rci.cpp:140-142,164 update
- * p and x every iteration): average time (ms) of the FP64 solver's x/p pass (k_flush_xp: four
- * iterations of p = s + beta p, x += alpha p from a ring of four s vectors, run every fourth
+ * p and x every iteration): average time (ms) of the FP64 solver's x/p pass (k_flush_xp: eight
+ * iterations of p = s + beta p, x += alpha p from eight s vectors of its ring, run every eighth
  * iteration) over the systems of the current batch, `reps` launches, and its algorithmic bytes
- * per launch (x, p and four s read, x and p written: 8 streams per system and edge). */
+ * per launch (x, p and eight s read, x and p written: 12 streams per system and edge). */
 int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes_per_launch);
 
 /* Calibration hook: average time (ms) of a plain device-to-device copy kernel (k_copy, 16-byte

@@ -170,7 +170,7 @@ class Solver:
         return ms.value, by.value
 
     def bench_flush(self, reps: int = 20):
-        """Average ms and algorithmic bytes of the FP64 solver's x/p pass (every 4th iteration)."""
+        """Average ms and algorithmic bytes of the FP64 solver's x/p pass (every 8th iteration)."""
         ms, by = C.c_double(), C.c_double()
         check(self._L.mtcocr_bench_flush(self._h, int(reps), C.byref(ms), C.byref(by)),
               "mtcocr_bench_flush")

@@ -87,10 +87,16 @@ struct SysScalars {
   int outer_status, outer, last_improve, stall_window;
   int fallback, fallback_iter, rollback, halt;
   int last, pad1, pad2, pad3;  // last: the iteration whose x is the answer (buffer last % 2)
-  // alpha (re, im) and beta (re, im) of iteration k at position (k - 1) % 8 of a chunk of 8:
-  // the FP64 solver applies the x/p recurrence four iterations at a time from these
-  double hab[8][4];
+  // alpha (re, im) and beta (re, im) of iteration k at position (k - 1) % 16 of a chunk of 16:
+  // the FP64 solver applies the x/p recurrence eight iterations at a time from these
+  double hab[16][4];
 };
+
+// The FP64 solver's x/p recurrence runs every kXpPeriod iterations from a ring of kRing s
+// buffers (s_{k-8}..s_{k-1} are read while the update pass has written s_k), and its graphs hold
+// kRing iterations, so that each position of the ring is a fixed buffer in them.
+constexpr int kXpPeriod = 8;
+constexpr int kRing = 2 * kXpPeriod;
 
 template <class T>
 struct EllView {
@@ -2303,27 +2309,29 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
 }
 
 
-// The x/p recurrence of four COCR iterations k-3..k in one pass (pcocr_rci.cpp:140-142,164:
-// p_j = s_{j-1} + beta_j p_{j-1}, x_j = x_{j-1} + alpha_j p_j), for the FP64 solver whose
-// update pass leaves x and p out (k_xr_fused<.., true>). Nothing else in an iteration reads x
-// or p, so they can lag; the update pass keeps s_{k-4}..s_{k-1} in a ring of eight buffers
-// (s_i below is s_{k-4+i}) and the scalar kernels record each iteration's alpha and beta
-// (SysScalars::hab, position (j - 1) % 8 = c0 + i). The arithmetic is that of the eager update
-// pass, in the same order, so x and p carry the same bits.
+// The x/p recurrence of kXpPeriod (8) COCR iterations k-7..k in one pass
+// (pcocr_rci.cpp:140-142,164: p_j = s_{j-1} + beta_j p_{j-1}, x_j = x_{j-1} + alpha_j p_j), for
+// the FP64 solver whose update pass leaves x and p out (k_xr_fused<.., true>). Nothing else in
+// an iteration reads x or p, so they can lag; the update pass keeps s_{k-8}..s_{k-1} in a ring
+// of sixteen buffers (ring.s[i] below is s_{k-8+i}) and the scalar kernels record each
+// iteration's alpha and beta (SysScalars::hab, position (j - 1) % 16 = c0 + i). The arithmetic
+// is that of the eager update pass, in the same order, so x and p carry the same bits.
 //
-// Runs after s_mrs_stop of iteration k (k = nrun[1], a multiple of 4), when every stop up to
+// Runs after s_mrs_stop of iteration k (k = nrun[1], a multiple of 8), when every stop up to
 // iteration k - 1 is known: a system that stopped at iteration `last` takes the steps j <= last
-// only, so x then holds its answer, x_last. A system still running takes all four; should it
+// only, so x then holds its answer, x_last. A system still running takes all eight; should it
 // stop at iteration k, x_k is its answer. Lanes with nothing to apply store x as loaded (their
 // dead p and s may hold anything). x and p are updated in place: each element is read and
 // written by one thread. It runs even when no system is running any more (at most twice per
 // solve), since the last stops of a batch complete their x here.
+struct XpRing {
+  const double2* s[kXpPeriod];
+};
+
 template <int SP>
 __global__ __launch_bounds__(kBlock) void k_flush_xp(
-    double2* __restrict__ x, double2* __restrict__ p, const double2* __restrict__ s0,
-    const double2* __restrict__ s1, const double2* __restrict__ s2,
-    const double2* __restrict__ s3, const SysScalars* __restrict__ sc,
-    const int* __restrict__ nrun, int c0, int nb) {
+    double2* __restrict__ x, double2* __restrict__ p, XpRing ring,
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, int c0, int nb) {
   const long long N = (long long)nb * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
   TileWalk tw(ntiles);
@@ -2331,54 +2339,44 @@ __global__ __launch_bounds__(kBlock) void k_flush_xp(
   const int k = nrun[1];
   const bool run = sc[s].status == ST_RUNNING;
   const int last = sc[s].last;
-  bool use[4];
-  double ar[4], ai[4], br[4], bi[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = k - 3 + i;
-    use[i] = j >= 1 && (run || j <= last);
-    const double* hq = sc[s].hab[(c0 + i) & 7];
-    ar[i] = hq[0];
-    ai[i] = hq[1];
-    br[i] = hq[2];
-    bi[i] = hq[3];
+  // alpha and beta of the eight iterations per system in LDS (in registers they took the
+  // kernel to 172 VGPRs), the steps a lane applies as a bit mask
+  __shared__ double2 hab_l[kXpPeriod][SP][2];
+  for (int q = threadIdx.x; q < kXpPeriod * SP; q += kBlock) {
+    const int i = q / SP, sy = q % SP;
+    const double* hq = sc[sy].hab[(c0 + i) & (kRing - 1)];
+    hab_l[i][sy][0] = make_double2(hq[0], hq[1]);
+    hab_l[i][sy][1] = make_double2(hq[2], hq[3]);
   }
-  struct Elem {
-    double2 x, p, s[4];
-  };
-  auto load = [&](long long t, Elem& E) {
-    E.x = ld_nt(x + t);
-    E.p = ld_nt(p + t);
-    E.s[0] = ld_nt(s0 + t);
-    E.s[1] = ld_nt(s1 + t);
-    E.s[2] = ld_nt(s2 + t);
-    E.s[3] = ld_nt(s3 + t);
-  };
-  auto apply = [&](long long t, Elem& E) {
-    double2 xv = E.x, pv = E.p;
+  unsigned use = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const double2 pn = caxpy(E.s[i], pv, br[i], bi[i]);
-      const double2 xn = caxpy(xv, pn, ar[i], ai[i]);
+  for (int i = 0; i < kXpPeriod; ++i) {
+    const int j = k - (kXpPeriod - 1) + i;
+    if (j >= 1 && (run || j <= last)) use |= 1u << i;
+  }
+  __syncthreads();
+  // one element per thread and step (x, p and eight s in flight)
+  for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
+    const long long t = tile * kBlock + threadIdx.x;
+    if (t >= N) continue;
+    double2 sv[kXpPeriod];
+    double2 xv = ld_nt(x + t), pv = ld_nt(p + t);
+#pragma unroll
+    for (int i = 0; i < kXpPeriod; ++i) sv[i] = ld_nt(ring.s[i] + t);
+#pragma unroll
+    for (int i = 0; i < kXpPeriod; ++i) {
+      const double2 al = hab_l[i][s][0], be = hab_l[i][s][1];
+      const double2 pn = caxpy(sv[i], pv, be.x, be.y);
+      const double2 xn = caxpy(xv, pn, al.x, al.y);
       // selects, not branches: every lane of a wave keeps issuing the same memory instructions
-      pv.x = use[i] ? pn.x : pv.x;
-      pv.y = use[i] ? pn.y : pv.y;
-      xv.x = use[i] ? xn.x : xv.x;
-      xv.y = use[i] ? xn.y : xv.y;
+      const bool u = (use >> i) & 1u;
+      pv.x = u ? pn.x : pv.x;
+      pv.y = u ? pn.y : pv.y;
+      xv.x = u ? xn.x : xv.x;
+      xv.y = u ? xn.y : xv.y;
     }
     st_nt(x + t, xv);
     st_nt(p + t, pv);
-  };
-  for (long long tile = tw.first; tile < tw.last; tile += 2 * tw.step) {
-    const long long t0 = tile * kBlock + threadIdx.x;
-    const long long t1 = (tile + tw.step) * kBlock + threadIdx.x;
-    const bool ok0 = t0 < N;
-    const bool ok1 = tile + tw.step < tw.last && t1 < N;
-    Elem e0, e1;
-    if (ok0) load(t0, e0);
-    if (ok1) load(t1, e1);
-    if (ok0) apply(t0, e0);
-    if (ok1) apply(t1, e1);
   }
 }
 
@@ -2466,7 +2464,7 @@ __global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, in
       cdiv(sc[s].zs_re, sc[s].zs_im, awr, awi, qr, qi);
       sc[s].al_re = qr;
       sc[s].al_im = qi;
-      double* hq = sc[s].hab[(sc[s].iter - 1) & 7];
+      double* hq = sc[s].hab[(sc[s].iter - 1) & (kRing - 1)];
       hq[0] = qr;
       hq[1] = qi;
       hq[2] = sc[s].be_re;
@@ -2845,7 +2843,7 @@ struct mtcocr_handle {
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
   DevBuf va2;  // the second buffer of a, alternating with va, for k_pc_grid
-  DevBuf vsr[7];  // s buffers 1..7 of the FP64 solver's ring (buffer 0 is vs)
+  DevBuf vsr[kRing - 1];  // s buffers 1..15 of the FP64 solver's ring (buffer 0 is vs)
   // the same vectors for a narrowed batch (swapped with the above by compaction)
   DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
   DevBuf cmap;  // slot maps of compaction / scatter
@@ -3198,10 +3196,10 @@ struct Engine {
   double *part, *part2;
   const double* omega;
   int nb, m, grid_s, grid_e, grid_n, grid_u, layout;
-  // the FP64 solver's ring of s buffers (s_k in sr[k % 8], sr[0] = s): its update pass leaves
-  // x and p to k_flush_xp, which reads s_{k-4}..s_{k-1} every fourth iteration
+  // the FP64 solver's ring of s buffers (s_k in sr[k % 16], sr[0] = s): its update pass leaves
+  // x and p to k_flush_xp, which reads s_{k-8}..s_{k-1} every eighth iteration
   static constexpr bool kLazy = std::is_same<T, double>::value;
-  T2* sr[8];
+  T2* sr[kRing];
   // the fused grid preconditioner (k_pc_grid): used, its view and grid
   bool use_pg = false;
   PgView pg;
@@ -3241,7 +3239,7 @@ struct Engine {
     layout = h->spmv_layout[q][k];
     a2 = h->va2.as<T2>();
     sr[0] = s;
-    for (int i = 1; i < 8; ++i) sr[i] = kLazy ? h->vsr[i - 1].as<T2>() : s;
+    for (int i = 1; i < kRing; ++i) sr[i] = kLazy ? h->vsr[i - 1].as<T2>() : s;
     use_pg = pg_applies(h, SP, sizeof(T2));
     if (use_pg) {
       pg = pg_view(h, h->pg_kc[q]);
@@ -3338,19 +3336,21 @@ struct Engine {
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     T2* s_next = s;  // s_k
     if constexpr (kLazy) {
-      s_next = sr[(c + 1) & 7];
+      s_next = sr[(c + 1) & (kRing - 1)];
       bool formed = false;  // w formed from help in the pass (after k_pc_grid)
       if constexpr (pg_width(SP)) {
         if (use_pg) {
           k_xr_fused<T, SP, true, true><<<grid_u, kBlock, 0, st>>>(
-              nullptr, nullptr, nullptr, r, acur, sr[c & 7], s_next, help, nullptr, sm, sc, nrun,
+              nullptr, nullptr, nullptr, r, acur, sr[c & (kRing - 1)], s_next, help, nullptr, sm,
+              sc, nrun,
               part, nb, wrec());
           formed = true;
         }
       }
       if (!formed)
         k_xr_fused<T, SP, true><<<grid_u, kBlock, 0, st>>>(nullptr, nullptr, nullptr, r, acur,
-                                                           sr[c & 7], s_next, w, nullptr, sm, sc,
+                                                           sr[c & (kRing - 1)], s_next, w, nullptr,
+                                                           sm, sc,
                                                            nrun, part, nb);
     } else {
       k_xr_fused<T, SP><<<grid_u, kBlock, 0, st>>>(odd ? x : x2, odd ? x2 : x, p, r, acur, s, s,
@@ -3359,12 +3359,14 @@ struct Engine {
     pp = partials<SP * 3>(st, grid_u, nr);
     s_mrs_stop<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     if constexpr (kLazy) {
-      // every fourth iteration (k = 4, 8, ...: positions 3 and 7), once the stops up to k - 1
-      // are known: x and p of iterations k-3..k from s_{k-4}..s_{k-1}
-      if ((c & 3) == 3)
-        k_flush_xp<SP><<<grid_u, kBlock, 0, st>>>(x, p, sr[(c - 3) & 7], sr[(c - 2) & 7],
-                                                  sr[(c - 1) & 7], sr[c & 7], sc, nrun,
-                                                  (c - 3) & 7, nb);
+      // every eighth iteration (k = 8, 16, ...: positions 7 and 15), once the stops up to
+      // k - 1 are known: x and p of iterations k-7..k from s_{k-8}..s_{k-1}
+      if ((c & (kXpPeriod - 1)) == kXpPeriod - 1) {
+        const int c0 = (c - (kXpPeriod - 1)) & (kRing - 1);
+        XpRing ring;
+        for (int i = 0; i < kXpPeriod; ++i) ring.s[i] = sr[(c0 + i) & (kRing - 1)];
+        k_flush_xp<SP><<<grid_u, kBlock, 0, st>>>(x, p, ring, sc, nrun, c0, nb);
+      }
     }
     spmv<0>(st, s_next, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
     pp = partials<SP * 2>(st, grid_s, nr);
@@ -3413,9 +3415,11 @@ template <class T, int SP>
 int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_below = 0) {
   hipStream_t st = h->stream;
   const int prec = std::is_same<T, double>::value ? 0 : 1;
-  // 8 iterations per graph: the FP64 solver's ring of s buffers and its x/p flushes (at
-  // positions 3 and 7) are laid out for chunks of 8 that start at iterations 1, 9, 17, ...
-  const int chunk = 8;
+  // kRing (16) iterations per graph in FP64: the solver's ring of s buffers and its x/p
+  // flushes (at positions 7 and 15) are laid out for chunks of 16 that start at iterations 1,
+  // 17, 33, ... (so compaction, between chunks, finds x and p complete and s_k in sr[0]); 8
+  // in FP32
+  const int chunk = Engine<T, SP>::kLazy ? kRing : 8;
   if (use_graphs()) {
     if (!(h->graph && h->graph_prec == prec && h->graph_sp == SP && h->graph_chunk == chunk &&
           h->graph_maxiter == maxiter && h->graph_pg == int(E.use_pg))) {
@@ -4534,9 +4538,9 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   if (rc) return rc;
   hipStream_t st = h->stream;
   std::vector<SysScalars> hs;
-  set_eps_status(h, hs, h->nsys, sp, 1e-8);  // running systems: every lane takes all four steps
+  set_eps_status(h, hs, h->nsys, sp, 1e-8);  // running systems: every lane takes all eight steps
   HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
-  const int kn[2] = {1, 4};  // [0] running count, [1] the iteration a flush ends
+  const int kn[2] = {1, kXpPeriod};  // [0] running count, [1] the iteration a flush ends
   HIPC(hipMemcpyAsync(h->nrun.p, kn, sizeof(kn), hipMemcpyHostToDevice, st));
   const long long nv = (long long)h->nb * sp;
   const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
@@ -4544,11 +4548,13 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
     Engine<double, SP> E(h);
-    for (double2* v : {E.x, E.p, E.sr[0], E.sr[1], E.sr[2], E.sr[3]})
-      k_fill<double2><<<gf, 256, 0, st>>>(v, nv);
+    XpRing ring;
+    for (int i = 0; i < kXpPeriod; ++i) ring.s[i] = E.sr[i];
+    k_fill<double2><<<gf, 256, 0, st>>>(E.x, nv);
+    k_fill<double2><<<gf, 256, 0, st>>>(E.p, nv);
+    for (int i = 0; i < kXpPeriod; ++i) k_fill<double2><<<gf, 256, 0, st>>>(E.sr[i], nv);
     auto launch = [&]() {
-      k_flush_xp<SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, E.sr[0], E.sr[1], E.sr[2], E.sr[3],
-                                                  E.sc, E.nrun, 0, E.nb);
+      k_flush_xp<SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, ring, E.sc, E.nrun, 0, E.nb);
     };
     launch();
     HIPC(hipEventRecord(h->ev0, st));
@@ -4561,8 +4567,8 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   });
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
-  // x, p and four s read, x and p written, once per system and edge
-  *bytes_per_launch = 8.0 * 16.0 * double(h->nb) * double(h->nsys);
+  // x, p and eight s read, x and p written, once per system and edge
+  *bytes_per_launch = double(kXpPeriod + 4) * 16.0 * double(h->nb) * double(h->nsys);
   return 0;
 }
 
