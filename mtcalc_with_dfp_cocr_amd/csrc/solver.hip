@@ -3307,9 +3307,10 @@ struct Engine {
     s_beta<SP, true><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun, maxiter);
   }
 
-  // one COCR iteration (stages 5-8) at position c = 0..7 of a chunk of 8 (iteration k with
-  // c = (k - 1) % 8). `odd` = k is odd: the eager update pass then reads x from buffer 0 and
-  // writes buffer 1, and k_pc_grid reads a and writes a2.
+  // one COCR iteration (stages 5-8) at position c of a graph chunk (c = 0..15 with
+  // c = (k - 1) % 16 for iteration k in FP64, c = 0..7 in FP32). `odd` = k is odd: the eager
+  // update pass then reads x from buffer 0 and writes buffer 1, and k_pc_grid reads a and
+  // writes a2.
   void iteration(hipStream_t st, int maxiter, int c) {
     const bool odd = (c & 1) == 0;
     int nr;
