@@ -255,8 +255,10 @@ int mtcocr_autotune(mtcocr_handle* h, int precision, int reps, int* cap_spmv, in
  * in the FP32-vector inner solve of the mixed solver also p = s + beta p, x += alpha p and the
  * smoothed x) over the systems of the current batch, `reps` launches on filled work vectors,
  * and the algorithmic bytes per launch (each stream read or written once per system and edge:
- * 8 streams in FP64, whose x and p are updated by k_flush_xp every eighth iteration; 14 with
- * the FP32 vectors). */
+ * 8 streams in FP64, whose x and p are updated by k_flush_xp every eighth iteration; in FP64
+ * on a box mesh in the grid order with the fused grid preconditioner, the pass that forms w
+ * from the node vector help: 7 streams plus help once per interior node and system and the
+ * diagonal once per edge; 14 streams with the FP32 vectors). */
 int mtcocr_bench_update(mtcocr_handle* h, int precision, int reps, double* avg_ms,
                         double* bytes_per_launch);
 
