@@ -727,6 +727,36 @@ def test_bench_hooks(M, c1_model, c1_mesh):
         assert all(q["req"] == 0 for q in st)
 
 
+def test_bench_hooks_grid_order(M, c1_model, c1_mesh):
+    """The same hooks on a box mesh in the grid order (width 8, fused grid preconditioner): the
+    FP64 update pass forms w from the node vector help, so it reports 7 vector streams plus
+    help once per interior node and system and the diagonal once per edge; the x/p pass 12
+    streams. The batch then solves as usual."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    say("bench hooks, grid order: 5 systems")
+    cases = [(f, a) for f in (0.1, 1.0, 10.0) for a in (1, 0)][:5]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    nx, ny, nz = c1_model.shape_nodes
+    m = (nx - 2) * (ny - 2) * (nz - 2)
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, ORDER_GRID)
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        B.set_rhs(prs, om)
+        ms, by = B.bench_update("fp64", 3)
+        assert ms > 0 and by == 7 * 16 * p.nb * len(cases) + 16 * m * len(cases) + 16 * p.nb
+        ms, by = B.bench_flush(3)
+        assert ms > 0 and by == 12 * 16 * p.nb * len(cases)
+        B.set_rhs(prs, om)
+        rc, st = B.solve_resident(1e-8, 10000, "fp64")
+        assert all(q["req"] == 0 for q in st)
+
+
 def test_wide_grid_two_stage_reduction(O, M, c1_model, c1_mesh):
     """16 systems on 45000 edges: at 16384 workgroups the dot-product partials (more than
     16384 doubles) are folded in two stages, at 256 they are reduced in one. Same solution to
