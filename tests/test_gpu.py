@@ -460,6 +460,40 @@ def test_iteration_limit_at_every_xp_position(O, M, c1_model, c1_mesh):
             assert d < 0.01 * gap
 
 
+def test_graph_replay_matches_direct_launches(O, M, c1_model, c1_mesh, monkeypatch):
+    """The FP64 solver's 16-iteration chunks replayed as HIP graphs give the bits of the same
+    kernels launched directly (MTCOCR_NO_GRAPH=1): a batch of 16 systems in the grid order
+    (stencil SpMV, fused preconditioner, w formed in the update pass, x and p every eighth
+    iteration, compaction to 8, 4, 2, 1), solved twice each way on one handle."""
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+    from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
+
+    freqs = M.frequencies(1e-2, 1e2, 8)
+    cases = [(f, a) for f in freqs for a in (1, 0)]
+    p, kd, kg, sd, sg = M.assemble_ks(c1_model)
+    prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
+    om = np.array([M.omega_of(f) for f, _ in cases])
+    runs = []
+    with Solver(0) as B:
+        B.set_mesh_order(c1_mesh, ORDER_GRID)
+        B.set_matrix_ks(p, kd, kg, sd, sg)
+        B.set_gradient(c1_mesh)
+        for direct in (False, True, False, True):
+            if direct:
+                monkeypatch.setenv("MTCOCR_NO_GRAPH", "1")
+            else:
+                monkeypatch.delenv("MTCOCR_NO_GRAPH", raising=False)
+            x, st = B.solve_batch(om, prs, eps=1e-8, precision="fp64")
+            runs.append((np.asarray(x), [q["iter"] for q in st], [q["req"] for q in st]))
+            say(f"{'direct' if direct else 'graphs'}: iters {runs[-1][1]}")
+    x0, it0, rq0 = runs[0]
+    assert all(r == 0 for r in rq0)
+    for x, it, rq in runs[1:]:
+        assert it == it0 and rq == rq0
+        assert np.array_equal(x.view(np.uint64), x0.view(np.uint64))
+
+
 def test_pcocr_2x2_folded_mirror(S, M, c1_system, c1_mesh):
     """The reference-signature entry point gives the same x as the handle API (deterministic)."""
     say("pcocr_2x2_folded mirror")
