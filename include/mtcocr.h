@@ -236,6 +236,16 @@ int mtcocr_get_stencil_sweep(mtcocr_handle* h, int precision, int* kc);
 int mtcocr_set_stencil_band(mtcocr_handle* h, int band);
 int mtcocr_get_stencil_band(mtcocr_handle* h, int precision, int* band);
 
+/* The FP64 solver's x/p period: x and p are updated every `period` iterations (8 or 16) from
+ * a ring of 2 * period s vectors (the reference updates them every iteration,
+ * pcocr_rci.cpp:140-142,164; the answer is the same bit for bit). 16 moves fewer bytes per
+ * iteration but holds 16 more batch vectors on the device; 0 (the default) takes 16 when, at
+ * the first FP64 solve of a batch shape, they leave 8 GiB and the compaction buffers free,
+ * else 8. set 8 frees the extra vectors. get reports the period in use. No counterpart in the
+ * reference. */
+int mtcocr_set_xp_period(mtcocr_handle* h, int period);
+int mtcocr_get_xp_period(mtcocr_handle* h, int* period);
+
 /* Autotuner: for the batch currently on the handle (set by mtcocr_set_rhs or a previous
  * mtcocr_solve_batch; needs the matrix and the gradient mesh) times the SpMV (all layouts),
  * node, edge and update-pass kernels at caps 512..131072, `reps` launches each, in

@@ -97,6 +97,8 @@ SIGNATURES = {
     "mtcocr_get_stencil_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mtcocr_set_stencil_band": (C.c_int, [C.c_void_p, C.c_int]),
     "mtcocr_get_stencil_band": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "mtcocr_set_xp_period": (C.c_int, [C.c_void_p, C.c_int]),
+    "mtcocr_get_xp_period": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mtcocr_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),

@@ -205,6 +205,15 @@ class Solver:
         """Rows a task of the stencil SpMVs sweeps along z (every width and precision)."""
         check(self._L.mtcocr_set_stencil_sweep(self._h, int(kc)), "mtcocr_set_stencil_sweep")
 
+    def set_xp_period(self, period: int) -> None:
+        """x/p period of the FP64 solver: 8 or 16 iterations, 0 to choose by free memory."""
+        check(self._L.mtcocr_set_xp_period(self._h, int(period)), "mtcocr_set_xp_period")
+
+    def get_xp_period(self) -> int:
+        v = C.c_int()
+        check(self._L.mtcocr_get_xp_period(self._h, C.byref(v)), "mtcocr_get_xp_period")
+        return v.value
+
     def set_stencil_band(self, band: int) -> None:
         """Tile columns per band of the LDS-staged stencil SpMV's task order (0: one band)."""
         check(self._L.mtcocr_set_stencil_band(self._h, int(band)), "mtcocr_set_stencil_band")

@@ -77,6 +77,13 @@ __device__ inline void st_nt(T2* p, T2 o) {
   __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
 }
 
+// The FP64 solver's x/p recurrence runs every P iterations (P = 8 or 16 per handle, by the
+// device memory free for its ring) from a ring of 2P s buffers (s_{k-P}..s_{k-1} are read
+// while the update pass has written s_k), and its graphs hold 2P iterations, so that each
+// position of the ring is a fixed buffer in them.
+constexpr int kXpMax = 16;
+constexpr int kRingMax = 2 * kXpMax;
+
 // Per-system scalar state (double precision in every mode).
 struct SysScalars {
   double zs_re, zs_im, zs1_re, zs1_im;
@@ -87,16 +94,10 @@ struct SysScalars {
   int outer_status, outer, last_improve, stall_window;
   int fallback, fallback_iter, rollback, halt;
   int last, pad1, pad2, pad3;  // last: the iteration whose x is the answer (buffer last % 2)
-  // alpha (re, im) and beta (re, im) of iteration k at position (k - 1) % 16 of a chunk of 16:
-  // the FP64 solver applies the x/p recurrence eight iterations at a time from these
-  double hab[16][4];
+  // alpha (re, im) and beta (re, im) of iteration k at position (k - 1) % 32: the FP64 solver
+  // applies the x/p recurrence eight or sixteen iterations at a time from these
+  double hab[kRingMax][4];
 };
-
-// The FP64 solver's x/p recurrence runs every kXpPeriod iterations from a ring of kRing s
-// buffers (s_{k-8}..s_{k-1} are read while the update pass has written s_k), and its graphs hold
-// kRing iterations, so that each position of the ring is a fixed buffer in them.
-constexpr int kXpPeriod = 8;
-constexpr int kRing = 2 * kXpPeriod;
 
 template <class T>
 struct EllView {
@@ -2309,29 +2310,30 @@ __global__ __launch_bounds__(kBlock) void k_xr_fused(
 }
 
 
-// The x/p recurrence of kXpPeriod (8) COCR iterations k-7..k in one pass
+// The x/p recurrence of NP (8 or 16) COCR iterations k-NP+1..k in one pass
 // (pcocr_rci.cpp:140-142,164: p_j = s_{j-1} + beta_j p_{j-1}, x_j = x_{j-1} + alpha_j p_j), for
 // the FP64 solver whose update pass leaves x and p out (k_xr_fused<.., true>). Nothing else in
-// an iteration reads x or p, so they can lag; the update pass keeps s_{k-8}..s_{k-1} in a ring
-// of sixteen buffers (ring.s[i] below is s_{k-8+i}) and the scalar kernels record each
-// iteration's alpha and beta (SysScalars::hab, position (j - 1) % 16 = c0 + i). The arithmetic
-// is that of the eager update pass, in the same order, so x and p carry the same bits.
+// an iteration reads x or p, so they can lag; the update pass keeps s_{k-NP}..s_{k-1} in a ring
+// of 2 NP buffers (ring.s[i] below is s_{k-NP+i}) and the scalar kernels record each
+// iteration's alpha and beta (SysScalars::hab, position (j - 1) % 32). The arithmetic is that
+// of the eager update pass, in the same order, so x and p carry the same bits.
 //
-// Runs after s_mrs_stop of iteration k (k = nrun[1], a multiple of 8), when every stop up to
+// Runs after s_mrs_stop of iteration k (k = nrun[1], a multiple of NP), when every stop up to
 // iteration k - 1 is known: a system that stopped at iteration `last` takes the steps j <= last
-// only, so x then holds its answer, x_last. A system still running takes all eight; should it
+// only, so x then holds its answer, x_last. A system still running takes all NP; should it
 // stop at iteration k, x_k is its answer. Lanes with nothing to apply store x as loaded (their
 // dead p and s may hold anything). x and p are updated in place: each element is read and
 // written by one thread. It runs even when no system is running any more (at most twice per
 // solve), since the last stops of a batch complete their x here.
 struct XpRing {
-  const double2* s[kXpPeriod];
+  const double2* s[kXpMax];
 };
 
-template <int SP>
+template <int SP, int NP>
 __global__ __launch_bounds__(kBlock) void k_flush_xp(
     double2* __restrict__ x, double2* __restrict__ p, XpRing ring,
-    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, int c0, int nb) {
+    const SysScalars* __restrict__ sc, const int* __restrict__ nrun, int nb) {
+  static_assert(NP == 8 || NP == 16, "x/p periods of 8 or 16 iterations");
   const long long N = (long long)nb * SP;
   const long long ntiles = (N + kBlock - 1) / kBlock;
   TileWalk tw(ntiles);
@@ -2339,19 +2341,19 @@ __global__ __launch_bounds__(kBlock) void k_flush_xp(
   const int k = nrun[1];
   const bool run = sc[s].status == ST_RUNNING;
   const int last = sc[s].last;
-  // alpha and beta of the eight iterations per system in LDS (in registers they took the
-  // kernel to 172 VGPRs), the steps a lane applies as a bit mask
-  __shared__ double2 hab_l[kXpPeriod][SP][2];
-  for (int q = threadIdx.x; q < kXpPeriod * SP; q += kBlock) {
+  // alpha and beta of the NP iterations per system in LDS (in registers, eight of them took
+  // the kernel to 172 VGPRs), the steps a lane applies as a bit mask
+  __shared__ double2 hab_l[NP][SP][2];
+  for (int q = threadIdx.x; q < NP * SP; q += kBlock) {
     const int i = q / SP, sy = q % SP;
-    const double* hq = sc[sy].hab[(c0 + i) & (kRing - 1)];
+    const double* hq = sc[sy].hab[(k - NP + i) & (kRingMax - 1)];  // iteration k - NP + 1 + i
     hab_l[i][sy][0] = make_double2(hq[0], hq[1]);
     hab_l[i][sy][1] = make_double2(hq[2], hq[3]);
   }
   unsigned use = 0;
 #pragma unroll
-  for (int i = 0; i < kXpPeriod; ++i) {
-    const int j = k - (kXpPeriod - 1) + i;
+  for (int i = 0; i < NP; ++i) {
+    const int j = k - (NP - 1) + i;
     if (j >= 1 && (run || j <= last)) use |= 1u << i;
   }
   __syncthreads();
@@ -2359,12 +2361,12 @@ __global__ __launch_bounds__(kBlock) void k_flush_xp(
   for (long long tile = tw.first; tile < tw.last; tile += tw.step) {
     const long long t = tile * kBlock + threadIdx.x;
     if (t >= N) continue;
-    double2 sv[kXpPeriod];
+    double2 sv[NP];
     double2 xv = ld_nt(x + t), pv = ld_nt(p + t);
 #pragma unroll
-    for (int i = 0; i < kXpPeriod; ++i) sv[i] = ld_nt(ring.s[i] + t);
+    for (int i = 0; i < NP; ++i) sv[i] = ld_nt(ring.s[i] + t);
 #pragma unroll
-    for (int i = 0; i < kXpPeriod; ++i) {
+    for (int i = 0; i < NP; ++i) {
       const double2 al = hab_l[i][s][0], be = hab_l[i][s][1];
       const double2 pn = caxpy(sv[i], pv, be.x, be.y);
       const double2 xn = caxpy(xv, pn, al.x, al.y);
@@ -2464,7 +2466,7 @@ __global__ __launch_bounds__(kScalarThreads) void s_alpha(const double* part, in
       cdiv(sc[s].zs_re, sc[s].zs_im, awr, awi, qr, qi);
       sc[s].al_re = qr;
       sc[s].al_im = qi;
-      double* hq = sc[s].hab[(sc[s].iter - 1) & (kRing - 1)];
+      double* hq = sc[s].hab[(sc[s].iter - 1) & (kRingMax - 1)];
       hq[0] = qr;
       hq[1] = qi;
       hq[2] = sc[s].be_re;
@@ -2843,7 +2845,10 @@ struct mtcocr_handle {
   // COCR work vectors (type depends on the precision of the last solve)
   DevBuf vr, vp, vs, vz, va, vw, vym, vsm, vhelp, vdn, vx, vx2, vb;
   DevBuf va2;  // the second buffer of a, alternating with va, for k_pc_grid
-  DevBuf vsr[kRing - 1];  // s buffers 1..15 of the FP64 solver's ring (buffer 0 is vs)
+  DevBuf vsr[kRingMax - 1];  // s buffers 1..31 of the FP64 solver's ring (buffer 0 is vs)
+  // the FP64 solver's x/p period: 0 chooses 8 or 16 by free device memory at the first FP64
+  // solve of a batch shape (mtcocr_set_xp_period), xp_period the one in use
+  int xp_request = 0, xp_period = 8;
   // the same vectors for a narrowed batch (swapped with the above by compaction)
   DevBuf cvr, cvp, cvs, cvz, cva, cvw, cvym, cvsm, cvx, cvx2;
   DevBuf cmap;  // slot maps of compaction / scatter
@@ -3199,7 +3204,8 @@ struct Engine {
   // the FP64 solver's ring of s buffers (s_k in sr[k % 16], sr[0] = s): its update pass leaves
   // x and p to k_flush_xp, which reads s_{k-8}..s_{k-1} every eighth iteration
   static constexpr bool kLazy = std::is_same<T, double>::value;
-  T2* sr[kRing];
+  T2* sr[kRingMax];
+  int xp = 8, ring = 16;  // x/p period and ring length of this handle
   // the fused grid preconditioner (k_pc_grid): used, its view and grid
   bool use_pg = false;
   PgView pg;
@@ -3239,7 +3245,9 @@ struct Engine {
     layout = h->spmv_layout[q][k];
     a2 = h->va2.as<T2>();
     sr[0] = s;
-    for (int i = 1; i < kRing; ++i) sr[i] = kLazy ? h->vsr[i - 1].as<T2>() : s;
+    xp = h->xp_period;
+    ring = 2 * xp;
+    for (int i = 1; i < kRingMax; ++i) sr[i] = kLazy && i < ring ? h->vsr[i - 1].as<T2>() : s;
     use_pg = pg_applies(h, SP, sizeof(T2));
     if (use_pg) {
       pg = pg_view(h, h->pg_kc[q]);
@@ -3337,12 +3345,12 @@ struct Engine {
     s_alpha<SP><<<1, kScalarThreads, 0, st>>>(pp, nr, sc, nrun);
     T2* s_next = s;  // s_k
     if constexpr (kLazy) {
-      s_next = sr[(c + 1) & (kRing - 1)];
+      s_next = sr[(c + 1) & (ring - 1)];
       bool formed = false;  // w formed from help in the pass (after k_pc_grid)
       if constexpr (pg_width(SP)) {
         if (use_pg) {
           k_xr_fused<T, SP, true, true><<<grid_u, kBlock, 0, st>>>(
-              nullptr, nullptr, nullptr, r, acur, sr[c & (kRing - 1)], s_next, help, nullptr, sm,
+              nullptr, nullptr, nullptr, r, acur, sr[c & (ring - 1)], s_next, help, nullptr, sm,
               sc, nrun,
               part, nb, wrec());
           formed = true;
@@ -3350,7 +3358,7 @@ struct Engine {
       }
       if (!formed)
         k_xr_fused<T, SP, true><<<grid_u, kBlock, 0, st>>>(nullptr, nullptr, nullptr, r, acur,
-                                                           sr[c & (kRing - 1)], s_next, w, nullptr,
+                                                           sr[c & (ring - 1)], s_next, w, nullptr,
                                                            sm, sc,
                                                            nrun, part, nb);
     } else {
@@ -3362,11 +3370,14 @@ struct Engine {
     if constexpr (kLazy) {
       // every eighth iteration (k = 8, 16, ...: positions 7 and 15), once the stops up to
       // k - 1 are known: x and p of iterations k-7..k from s_{k-8}..s_{k-1}
-      if ((c & (kXpPeriod - 1)) == kXpPeriod - 1) {
-        const int c0 = (c - (kXpPeriod - 1)) & (kRing - 1);
-        XpRing ring;
-        for (int i = 0; i < kXpPeriod; ++i) ring.s[i] = sr[(c0 + i) & (kRing - 1)];
-        k_flush_xp<SP><<<grid_u, kBlock, 0, st>>>(x, p, ring, sc, nrun, c0, nb);
+      if ((c & (xp - 1)) == xp - 1) {
+        const int c0 = (c - (xp - 1)) & (ring - 1);
+        XpRing rs{};
+        for (int i = 0; i < xp; ++i) rs.s[i] = sr[(c0 + i) & (ring - 1)];
+        if (xp == 16)
+          k_flush_xp<SP, 16><<<grid_u, kBlock, 0, st>>>(x, p, rs, sc, nrun, nb);
+        else
+          k_flush_xp<SP, 8><<<grid_u, kBlock, 0, st>>>(x, p, rs, sc, nrun, nb);
       }
     }
     spmv<0>(st, s_next, z, (const T2*)nullptr, (T2*)nullptr, (T2*)nullptr);
@@ -3395,10 +3406,29 @@ int alloc_work(mtcocr_handle* h, int prec, int sp) {
   // FP64 batches run in groups of 16)
   if (sp >= 8 && h->pg_ok && (rc = ensure(h, h->va2, (size_t)h->nb * std::min(sp, 16) * es)))
     return rc;
-  // the FP64 solver's ring of s buffers (Engine::sr)
-  if (prec == MTCOCR_PREC_FP64)
-    for (DevBuf& b : h->vsr)
-      if ((rc = ensure(h, b, nv))) return rc;
+  // the FP64 solver's ring of s buffers (Engine::sr): 16 for an x/p period of 8; 32 for 16
+  // when asked for, or by default when the 16 more vectors leave the device 8 GiB and the
+  // compaction buffers (five vectors' worth) free
+  if (prec == MTCOCR_PREC_FP64) {
+    for (int i = 0; i < 15; ++i)
+      if ((rc = ensure(h, h->vsr[i], nv))) return rc;
+    int period = h->xp_request;
+    if (period == 0) {
+      period = 8;
+      if (h->vsr[15].bytes >= nv) {
+        period = 16;  // held from an earlier solve
+      } else {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > 21 * nv + (size_t(8) << 30))
+          period = 16;
+      }
+    }
+    if (period == 16)
+      for (int i = 15; i < kRingMax - 1; ++i)
+        if ((rc = ensure(h, h->vsr[i], nv))) return rc;
+    if (period != h->xp_period) drop_graph(h);  // captured chunks have the old length
+    h->xp_period = period;
+  }
   if ((rc = ensure(h, h->vhelp, nm))) return rc;
   if ((rc = ensure(h, h->vdn, nm))) return rc;
   h->work_prec = prec;
@@ -3416,11 +3446,11 @@ template <class T, int SP>
 int run_iterations(mtcocr_handle* h, Engine<T, SP>& E, int maxiter, int stop_below = 0) {
   hipStream_t st = h->stream;
   const int prec = std::is_same<T, double>::value ? 0 : 1;
-  // kRing (16) iterations per graph in FP64: the solver's ring of s buffers and its x/p
-  // flushes (at positions 7 and 15) are laid out for chunks of 16 that start at iterations 1,
-  // 17, 33, ... (so compaction, between chunks, finds x and p complete and s_k in sr[0]); 8
-  // in FP32
-  const int chunk = Engine<T, SP>::kLazy ? kRing : 8;
+  // 2P iterations per graph in FP64 (P = the x/p period, 8 or 16): the solver's ring of s
+  // buffers and its x/p flushes (at positions P - 1 and 2P - 1) are laid out for chunks of 2P
+  // that start at iterations 1, 2P + 1, ... (so compaction, between chunks, finds x and p
+  // complete and s_k in sr[0]); 8 in FP32
+  const int chunk = Engine<T, SP>::kLazy ? 2 * h->xp_period : 8;
   if (use_graphs()) {
     if (!(h->graph && h->graph_prec == prec && h->graph_sp == SP && h->graph_chunk == chunk &&
           h->graph_maxiter == maxiter && h->graph_pg == int(E.use_pg))) {
@@ -4541,7 +4571,8 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   std::vector<SysScalars> hs;
   set_eps_status(h, hs, h->nsys, sp, 1e-8);  // running systems: every lane takes all eight steps
   HIPC(hipMemcpyAsync(h->scal.p, hs.data(), sizeof(SysScalars) * sp, hipMemcpyHostToDevice, st));
-  const int kn[2] = {1, kXpPeriod};  // [0] running count, [1] the iteration a flush ends
+  const int xp = h->xp_period;
+  const int kn[2] = {1, xp};  // [0] running count, [1] the iteration a flush ends
   HIPC(hipMemcpyAsync(h->nrun.p, kn, sizeof(kn), hipMemcpyHostToDevice, st));
   const long long nv = (long long)h->nb * sp;
   const int gf = (int)std::min<long long>(2048, (nv + 255) / 256);
@@ -4549,13 +4580,16 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   rc = with_sp(sp, [&](auto spc) -> int {
     constexpr int SP = decltype(spc)::value;
     Engine<double, SP> E(h);
-    XpRing ring;
-    for (int i = 0; i < kXpPeriod; ++i) ring.s[i] = E.sr[i];
+    XpRing rs{};
+    for (int i = 0; i < xp; ++i) rs.s[i] = E.sr[i];
     k_fill<double2><<<gf, 256, 0, st>>>(E.x, nv);
     k_fill<double2><<<gf, 256, 0, st>>>(E.p, nv);
-    for (int i = 0; i < kXpPeriod; ++i) k_fill<double2><<<gf, 256, 0, st>>>(E.sr[i], nv);
+    for (int i = 0; i < xp; ++i) k_fill<double2><<<gf, 256, 0, st>>>(E.sr[i], nv);
     auto launch = [&]() {
-      k_flush_xp<SP><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, ring, E.sc, E.nrun, 0, E.nb);
+      if (xp == 16)
+        k_flush_xp<SP, 16><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, rs, E.sc, E.nrun, E.nb);
+      else
+        k_flush_xp<SP, 8><<<E.grid_u, kBlock, 0, st>>>(E.x, E.p, rs, E.sc, E.nrun, E.nb);
     };
     launch();
     HIPC(hipEventRecord(h->ev0, st));
@@ -4569,7 +4603,7 @@ int mtcocr_bench_flush(mtcocr_handle* h, int reps, double* avg_ms, double* bytes
   if (rc) return rc;
   *avg_ms = double(ms) / reps;
   // x, p and eight s read, x and p written, once per system and edge
-  *bytes_per_launch = double(kXpPeriod + 4) * 16.0 * double(h->nb) * double(h->nsys);
+  *bytes_per_launch = double(xp + 4) * 16.0 * double(h->nb) * double(h->nsys);
   return 0;
 }
 
@@ -4636,6 +4670,23 @@ int mtcocr_set_stencil_band(mtcocr_handle* h, int band) {
   for (auto& row : h->stl_band)
     for (int& c : row) c = band;
   drop_graph(h);  // captured launches hold the old task order
+  return 0;
+}
+
+int mtcocr_set_xp_period(mtcocr_handle* h, int period) {
+  if (!h || (period != 0 && period != 8 && period != 16)) return MTCOCR_ERR_ARG;
+  h->xp_request = period;
+  if (period == 8) {
+    if (h->xp_period != 8) drop_graph(h);  // captured chunks name the buffers freed below
+    for (int i = 15; i < kRingMax - 1; ++i) h->vsr[i].release();
+    h->xp_period = 8;
+  }
+  return 0;
+}
+
+int mtcocr_get_xp_period(mtcocr_handle* h, int* period) {
+  if (!h || !period) return MTCOCR_ERR_ARG;
+  *period = h->xp_period;
   return 0;
 }
 

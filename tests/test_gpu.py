@@ -423,12 +423,13 @@ def test_special_cases(S, M, c1_system, c1_mesh):
         assert st["req"] == -2 and st["iter"] == 1
 
 
-def test_iteration_limit_at_every_xp_position(O, M, c1_model, c1_mesh):
-    """The FP64 solver applies x and p eight iterations at a time; a system that halts at its
-    iteration limit returns x of the last iteration whatever its position in that window. A
-    batch of 16 systems (stencil SpMV, fused preconditioner) halted at maxiter = 57..65: each
-    checked system's x against the oracle's x at the same limit, closer to it than to the
-    oracle's x one iteration before or after by a factor of 100."""
+@pytest.mark.parametrize("period", [8, 16])
+def test_iteration_limit_at_every_xp_position(O, M, c1_model, c1_mesh, period):
+    """The FP64 solver applies x and p `period` iterations at a time; a system that halts at
+    its iteration limit returns x of the last iteration whatever its position in that window.
+    A batch of 16 systems (stencil SpMV, fused preconditioner) halted at maxiter = 57 ..
+    57 + period: each checked system's x against the oracle's x at the same limit, closer to it
+    than to the oracle's x one iteration before or after by a factor of 100."""
     if not gpu_available():
         pytest.skip("no GPU visible")
     from mtcalc_with_dfp_cocr_amd.cocr import ORDER_GRID, Solver
@@ -438,14 +439,16 @@ def test_iteration_limit_at_every_xp_position(O, M, c1_model, c1_mesh):
     p, kd, kg, sd, sg = M.assemble_ks(c1_model)
     prs = np.stack([M.assemble_rhs(c1_model, f, a) for f, a in cases])
     om = np.array([M.omega_of(f) for f, _ in cases])
-    limits = tuple(range(57, 66))
+    limits = tuple(range(57, 58 + period))
     gpu = {}
     with Solver(0) as B:
         B.set_mesh_order(c1_mesh, ORDER_GRID)
         B.set_matrix_ks(p, kd, kg, sd, sg)
         B.set_gradient(c1_mesh)
+        B.set_xp_period(period)
         for mi in limits:
             gpu[mi] = B.solve_batch(om, prs, eps=1e-8, maxiter=mi, precision="fp64")
+        assert B.get_xp_period() == period
     for k in (0, 5, 10, 15):
         f, a = cases[k]
         s = M.assemble(c1_model, f, a)
@@ -734,7 +737,8 @@ def test_launch_shapes_and_autotune(O, M, c1_model, c1_mesh):
 def test_bench_hooks(M, c1_model, c1_mesh):
     """Kernel timing hooks on a batch of 5 systems (width 8): SpMV, the fused update pass and
     the FP64 x/p pass report positive times and their algorithmic bytes (the FP64 update pass
-    moves 8 vectors, the FP32-vector one 14, the x/p pass 12); the resident batch is left
+    moves 8 vectors, the FP32-vector one 14, the x/p pass 12 or 20 at periods of 8 and 16); the
+    resident batch is left
     usable."""
     if not gpu_available():
         pytest.skip("no GPU visible")
@@ -754,8 +758,12 @@ def test_bench_hooks(M, c1_model, c1_mesh):
             assert ms > 0 and by == streams * elem * p.nb * len(cases)
             ms, by = B.bench_spmv(prec, 3)
             assert ms > 0 and by > 0
+        B.set_xp_period(8)
         ms, by = B.bench_flush(3)
-        assert ms > 0 and by == 12 * 16 * p.nb * len(cases)
+        assert B.get_xp_period() == 8 and ms > 0 and by == 12 * 16 * p.nb * len(cases)
+        B.set_xp_period(16)
+        ms, by = B.bench_flush(3)
+        assert B.get_xp_period() == 16 and ms > 0 and by == 20 * 16 * p.nb * len(cases)
         B.set_rhs(prs, om)
         rc, st = B.solve_resident(1e-8, 10000, "fp64")
         assert all(q["req"] == 0 for q in st)
@@ -785,7 +793,8 @@ def test_bench_hooks_grid_order(M, c1_model, c1_mesh):
         ms, by = B.bench_update("fp64", 3)
         assert ms > 0 and by == 7 * 16 * p.nb * len(cases) + 16 * m * len(cases) + 16 * p.nb
         ms, by = B.bench_flush(3)
-        assert ms > 0 and by == 12 * 16 * p.nb * len(cases)
+        xp = B.get_xp_period()
+        assert xp in (8, 16) and ms > 0 and by == (xp + 4) * 16 * p.nb * len(cases)
         B.set_rhs(prs, om)
         rc, st = B.solve_resident(1e-8, 10000, "fp64")
         assert all(q["req"] == 0 for q in st)
