@@ -15,7 +15,7 @@ synchronisations). Reported beside it:
                  systems whose refinement stalls
 The roofline is that of the dominant kernel per iteration (the longer of the batched FP64
 update pass and the batched FP64 SpMV), timed with HIP events on the solver's stream; the SpMV,
-the update pass and the x/p pass (every eighth iteration) are each reported with their HBM
+the update pass and the x/p pass (every 8th or 16th iteration) are each reported with their HBM
 traffic from rocprofv3 counters. The CPU baseline is the oracle (FP64 CPU restatement of the
 reference's COCR_FP) run to the stop test on one system of the same workload, on the host's
 cores.
@@ -236,7 +236,7 @@ def measure_traffic(args, sp):
         return None
     pats = {"spmv": re.compile(r"k_spmv(_rg|_blk|_stc|_stl)?<double, %d, 0>" % sp),
             "update": re.compile(r"k_xr_fused<double, %d, true(, true)?>" % sp),
-            "flush": re.compile(r"k_flush_xp<%d>" % sp)}
+            "flush": re.compile(r"k_flush_xp<%d(, \d+)?>" % sp)}
     res = {k: {} for k in pats}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="mtcocr_pmc_", dir="/tmp")
@@ -476,6 +476,7 @@ def main():
     achu = gbs(upd_bytes, upd_ms)
     log(f"update pass fp64 x{nsys}: {upd_ms:.3f} ms {achu:.0f} GB/s")
     xp_ms, xp_bytes = S.bench_flush(args.spmv_reps)
+    xp_period = S.get_xp_period()
     log(f"x/p pass fp64 x{nsys}: {xp_ms:.3f} ms {gbs(xp_bytes, xp_ms):.0f} GB/s")
     S.close()
 
@@ -574,7 +575,8 @@ def main():
     upd_name = f"k_xr_fused<double,{spb},true{',true' if args.order_block == -1 else ''}>"
     upd_line = kernel_line(f"{upd_name} (batched FP64 update pass)",
                            upd_bytes, upd_ms, traffic["update"] if traffic else None)
-    xp_line = kernel_line(f"k_flush_xp<{spb}> (batched FP64 x/p pass, every 8th iteration)",
+    xp_line = kernel_line(f"k_flush_xp<{spb},{xp_period}> (batched FP64 x/p pass, every "
+                          f"{xp_period}th iteration)",
                           xp_bytes, xp_ms, traffic["flush"] if traffic else None)
     # the dominant kernel: both run once per iteration, the longer one
     roof = upd_line if upd_ms > spmv_ms else spmv_line
